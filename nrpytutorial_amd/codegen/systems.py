@@ -1,0 +1,281 @@
+"""Generate the system bundles, calling the codegen API the way the reference's
+Start-to-Finish drivers do.
+
+* ``scalarwave_cartesian``: Tutorial-Start_to_Finish-ScalarWave.ipynb
+  (rhs_eval from ScalarWave_RHSs, plane-wave initial data, quadratic
+  extrapolation outer boundaries).
+* ``bssn_<coord>``: Tutorial-Start_to_Finish-BSSNCurvilinear-Two_BHs_Collide.ipynb:
+  BSSN_RHSs + gauge RHSs with LeaveRicciSymbolic and upwinding on beta^i
+  (:436-484), Ricci_eval (:486-540), Hamiltonian_constraint (:542-556),
+  enforce_detgammabar_constraint (Enforce_Detgammabar_Constraint.py:60-75),
+  Brill-Lindquist initial_data (BSSN_ID_function_string.py:9-39), rfm
+  precompute on, symmetry axis "2" for the axisymmetric Spherical runs.
+* ``maxwell_cartesian``: VacuumMaxwell_Flat_Evol_Cartesian System II.
+
+Each bundle is generated in a fresh interpreter (the registries are global):
+``python -m nrpytutorial_amd.codegen.systems <bundle>``.
+"""
+import sys
+import time
+import sympy as sp
+
+from .. import NRPy_param_funcs as par
+from .. import grid as gri
+from .. import reference_metric as rfm
+from .. import finite_difference as fin
+from .. import outputC as oc
+from ..outputC import lhrh
+from . import bundle as bd
+
+BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2", "maxwell_cartesian")
+
+
+def _mol_glue(has_ricci, has_detg, rhs_args_decl, rhs_call_args):
+    """MoL_step_forward_in_time + hipGraph replay for the diagonal RK4 schedule."""
+    ricci = ("  if ((rc = Ricci_eval(rfmstruct, params, IN, auxevol_gfs, stream))) return rc;\n" if has_ricci else "")
+    detg = ("  if ((rc = enforce_detgammabar_constraint(rfmstruct, params, OUT, stream))) return rc;\n"
+            if has_detg else "")
+    sub = ("  /* substep S: IN -> OUT */\n" + ricci +
+           "  if ((rc = rhs_eval_rk4_substep(S, " + rhs_call_args + ", y_n_gfs, y_nplus1_running_total_gfs, OUT, dt, "
+           "stream))) return rc;\n"
+           "  if ((rc = apply_bcs_curvilinear_composed(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, OUT, stream))) "
+           "return rc;\n" + detg)
+    steps = ""
+    for s, (i, o) in enumerate((("y_n_gfs", "k_odd_gfs"), ("k_odd_gfs", "k_even_gfs"),
+                                ("k_even_gfs", "k_odd_gfs"), ("k_odd_gfs", "y_n_gfs"))):
+        steps += sub.replace("IN", i).replace("OUT", o).replace("substep S", "substep %d" % s) \
+            .replace("(S,", "(%d," % s)
+    sig = ("const rfm_struct *rfmstruct, const paramstruct *params, const bc_struct *bcstruct, double *y_n_gfs, "
+           "double *k_odd_gfs, double *k_even_gfs, double *y_nplus1_running_total_gfs, double *auxevol_gfs, "
+           "const double dt")
+    return (
+        "// One RK4 step (MoLtimestepping/C_Code_Generation.py:266-329 for RK4): four substeps, each\n"
+        "// [Ricci] -> RHS + update at interior points -> ghost fill -> [det(gammabar) enforcement].\n"
+        'extern "C" int MoL_step_forward_in_time(' + sig + ", hipStream_t stream) {\n"
+        "  if (!rfmstruct || !params || !bcstruct || !y_n_gfs || !k_odd_gfs || !k_even_gfs ||\n"
+        "      !y_nplus1_running_total_gfs) return NRPY_ERR_NULL;\n"
+        + ("  if (!auxevol_gfs) return NRPY_ERR_NULL;\n" if has_ricci else "") +
+        "  int rc;\n" + steps + "  return NRPY_OK;\n}\n\n"
+        "struct MoL_graph {\n  hipGraph_t graph;\n  hipGraphExec_t exec;\n};\n"
+        "// Capture one RK4 step into a hipGraph; replay with MoL_graph_launch. Pointers and dt are fixed\n"
+        "// at capture time.\n"
+        'extern "C" int MoL_graph_create(' + sig + ", MoL_graph **out) {\n"
+        "  if (!out) return NRPY_ERR_NULL;\n  *out = nullptr;\n"
+        "  hipStream_t s;\n"
+        "  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return NRPY_ERR_LAUNCH;\n"
+        "  MoL_graph *g = new MoL_graph();\n"
+        "  int rc = NRPY_OK;\n"
+        "  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = NRPY_ERR_LAUNCH;\n"
+        "  if (!rc) {\n"
+        "    rc = MoL_step_forward_in_time(rfmstruct, params, bcstruct, y_n_gfs, k_odd_gfs, k_even_gfs,\n"
+        "                                  y_nplus1_running_total_gfs, auxevol_gfs, dt, s);\n"
+        "    hipGraph_t gr;\n"
+        "    const hipError_t e = hipStreamEndCapture(s, &gr);\n"
+        "    if (!rc && e != hipSuccess) rc = NRPY_ERR_LAUNCH;\n"
+        "    if (!rc) {\n"
+        "      g->graph = gr;\n"
+        "      if (hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0) != hipSuccess) rc = NRPY_ERR_LAUNCH;\n"
+        "    }\n"
+        "  }\n"
+        "  (void)hipStreamDestroy(s);\n"
+        "  if (rc) { delete g; return rc; }\n"
+        "  *out = g;\n  return NRPY_OK;\n}\n"
+        'extern "C" int MoL_graph_launch(MoL_graph *g, const int nsteps, hipStream_t stream) {\n'
+        "  if (!g) return NRPY_ERR_NULL;\n  if (nsteps < 0) return NRPY_ERR_PARAM;\n"
+        "  for (int n = 0; n < nsteps; n++)\n"
+        "    if (hipGraphLaunch(g->exec, stream) != hipSuccess) return NRPY_ERR_LAUNCH;\n"
+        "  return NRPY_OK;\n}\n"
+        'extern "C" void MoL_graph_free(MoL_graph *g) {\n'
+        "  if (!g) return;\n  (void)hipGraphExecDestroy(g->exec);\n  (void)hipGraphDestroy(g->graph);\n  delete g;\n}\n")
+
+
+def _fused_launcher(name, args_sig, call_args, loopopts):
+    rng_interior = "InteriorPoints" in loopopts
+    n = ("params->Nxx0", "params->Nxx1", "params->Nxx2") if rng_interior else \
+        ("params->Nxx_plus_2NGHOSTS0", "params->Nxx_plus_2NGHOSTS1", "params->Nxx_plus_2NGHOSTS2")
+    bx, by = bd.lp.BLOCK_X, bd.lp.BLOCK_Y
+    cases = "".join(
+        "    case %d: hipLaunchKernelGGL(%s__gpu<%d>, grid, block, 0, stream, %s, y_n, rt, out, dt); break;\n"
+        % (s, name, s, call_args) for s in range(4))
+    return ('extern "C" int %s(const int S, %s, double *y_n, double *rt, double *out, const double dt, '
+            "hipStream_t stream) {\n"
+            "  if (!params || !y_n || !rt || !out) return NRPY_ERR_NULL;\n"
+            "  if (S < 0 || S > 3) return NRPY_ERR_PARAM;\n"
+            "  if (nrpy_check_params(params)) return NRPY_ERR_SHAPE;\n"
+            "  const dim3 block(%d, %d, 1);\n"
+            "  const dim3 grid((%s + %d - 1) / %d, (%s + %d - 1) / %d, %s);\n"
+            "  switch (S) {\n%s  }\n"
+            "  return hipGetLastError() == hipSuccess ? 0 : NRPY_ERR_LAUNCH;\n}\n"
+            % (name, args_sig, bx, by, n[0], bx, bx, n[1], by, by, n[2], cases))
+
+
+def _header(NGHOSTS):
+    ps, members = bd.paramstruct_decl()
+    defines, lists = bd.gf_defines_and_meta()
+    parity = [bd.parity_type(n) for n in lists["evol"]]
+    pre = ('#include "../nrpy_runtime_pre.h"\n#define NGHOSTS %d\n' % NGHOSTS + defines + "\n" + ps + "\n"
+           + "static const int8_t evol_gf_parity[%d] = {%s};\n" % (max(1, len(parity)),
+                                                                   ", ".join(str(p) for p in parity) or "0")
+           + bd.check_params_code())
+    meta = {"NGHOSTS": NGHOSTS, "paramstruct": members, "gfs": lists, "evol_parity": parity,
+            "aux_parity": [bd.parity_type(n) for n in lists["aux"]],
+            "auxevol_parity": [bd.parity_type(n) for n in lists["auxevol"]]}
+    return pre, meta
+
+
+def _initial_data_kernel(rhss, gfnames, loopopts="AllPoints,Read_xxs"):
+    lhss = ["in_gfs[IDX4(%sGF, i0, i1, i2)]" % n.upper() for n in gfnames]
+    pairs = sorted(zip(lhss, rhss), key=lambda p: p[0])
+    body = oc.outputC([p[1] for p in pairs], [p[0] for p in pairs], "returnstring",
+                      params="includebraces=False,outCverbose=False")
+    return oc.outCfunction(outfile="returnstring", desc="Set up the initial data at all points on the numerical grid.",
+                           name="initial_data",
+                           params="const paramstruct *restrict params,REAL *restrict xx[3], REAL *restrict in_gfs",
+                           body=body, loopopts=loopopts)
+
+
+def scalarwave_cartesian():
+    from ..ScalarWave import ScalarWave_RHSs as sw, InitialData as swid
+    par.set_parval_from_str("grid::DIM", 3)
+    par.set_parval_from_str("finite_difference::FD_CENTDERIVS_ORDER", 4)
+    par.set_parval_from_str("reference_metric::CoordSystem", "Cartesian")
+    rfm.reference_metric()
+    sw.ScalarWave_RHSs()
+    swid.InitialData(Type="PlaneWave", CoordSystem="Cartesian")
+    exprs = [lhrh(lhs=gri.gfaccess("rhs_gfs", "uu"), rhs=sw.uu_rhs),
+             lhrh(lhs=gri.gfaccess("rhs_gfs", "vv"), rhs=sw.vv_rhs)]
+    body = fin.FD_outputC("returnstring", exprs, params="outCverbose=False")
+    rhs_src = oc.outCfunction(outfile="returnstring", desc="Evaluate the scalar wave RHSs", name="rhs_eval",
+                              params="const paramstruct *restrict params,const REAL *restrict in_gfs,"
+                                     "REAL *restrict rhs_gfs", body=body, loopopts="InteriorPoints")
+    args_decl = "const paramstruct params, const double *__restrict__ in_gfs"
+    fused = bd.fused_rk4_kernel("rhs_eval_rk4_substep", args_decl, "", body, loopopts="InteriorPoints")
+    fused += _fused_launcher("rhs_eval_rk4_substep", "const rfm_struct *rfmstruct, const paramstruct *params, "
+                             "const double *auxevol_gfs, const double *in_gfs", "*params, in_gfs", "InteriorPoints")
+    id_src = _initial_data_kernel([swid.uu_ID, swid.vv_ID], ["uu", "vv"])
+    pre, meta = _header(3)
+    parts = [pre, bd.coord_host_functions("Cartesian"), '#include "../nrpy_runtime.h"', bd.rfm_struct_code(),
+             rhs_src, fused, id_src,
+             _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN")]
+    meta.update({"bundle": "scalarwave_cartesian", "CoordSystem": "Cartesian", "rfm_tables": [],
+                 "kernels": ["rhs_eval", "initial_data"], "has_ricci": False, "has_detg": False})
+    return bd.write_bundle("scalarwave_cartesian", parts, meta)
+
+
+def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
+    from ..BSSN import BSSN_quantities as Bq, BSSN_RHSs as rhs, BSSN_gauge_RHSs as gaugerhs
+    from ..BSSN import Enforce_Detgammabar_Constraint as EGC, BSSN_constraints as bssncon
+    from ..BSSN import BrillLindquist as bl
+    t0 = time.time()
+    par.set_parval_from_str("grid::DIM", 3)
+    par.set_parval_from_str("finite_difference::FD_CENTDERIVS_ORDER", FD_order)
+    par.set_parval_from_str("reference_metric::CoordSystem", CoordSystem)
+    par.set_parval_from_str("indexedexp::symmetry_axes", symmetry_axes)
+    rfm.reference_metric()
+    # Initial data is written in closed form (Read_xxs), before rfm precompute is switched on.
+    bl.BrillLindquist()
+    id_rhss = [bl.trK, bl.alpha, bl.cf] + [v for i in range(3) for v in (bl.lambdaU[i], bl.vetU[i], bl.betU[i])] + \
+        [v for i in range(3) for j in range(i, 3) for v in (bl.hDD[i][j], bl.aDD[i][j])]
+    id_names = ["trK", "alpha", "cf"] + [v for i in range(3) for v in ("lambdaU%d" % i, "vetU%d" % i, "betU%d" % i)] + \
+        [v for i in range(3) for j in range(i, 3) for v in ("hDD%d%d" % (i, j), "aDD%d%d" % (i, j))]
+
+    par.set_parval_from_str("BSSN.BSSN_gauge_RHSs::ShiftEvolutionOption", "GammaDriving2ndOrder_Covariant")
+    par.set_parval_from_str("reference_metric::enable_rfm_precompute", "True")
+    rfm.reference_metric()
+    par.set_parval_from_str("BSSN.BSSN_quantities::LeaveRicciSymbolic", "True")
+    rhs.BSSN_RHSs()
+    gaugerhs.BSSN_gauge_RHSs()
+    Bq.BSSN_basic_tensors()
+    betaU = Bq.betaU
+    detg_exprs = EGC.Enforce_Detgammabar_Constraint_symb_expressions()
+    par.set_parval_from_str("BSSN.BSSN_quantities::LeaveRicciSymbolic", "False")
+    Bq.RicciBar__gammabarDD_dHatD__DGammaUDD__DGammaU()
+    RbarDD = Bq.RbarDD
+    gri.register_gridfunctions("AUX", "H")
+    bssncon.BSSN_constraints(add_T4UUmunu_source_terms=False)
+    print("(BENCH) symbolic expressions %.1fs" % (time.time() - t0), flush=True)
+
+    lhs_names = ["alpha", "cf", "trK"]
+    rhs_exprs = [gaugerhs.alpha_rhs, rhs.cf_rhs, rhs.trK_rhs]
+    for i in range(3):
+        lhs_names += ["betU%d" % i, "lambdaU%d" % i, "vetU%d" % i]
+        rhs_exprs += [gaugerhs.bet_rhsU[i], rhs.lambda_rhsU[i], gaugerhs.vet_rhsU[i]]
+        for j in range(i, 3):
+            lhs_names += ["aDD%d%d" % (i, j), "hDD%d%d" % (i, j)]
+            rhs_exprs += [rhs.a_rhsDD[i][j], rhs.h_rhsDD[i][j]]
+    pairs = sorted(zip(lhs_names, rhs_exprs), key=lambda p: p[0])
+    evol_rhss = [lhrh(lhs=gri.gfaccess("rhs_gfs", n), rhs=e) for n, e in pairs]
+    t0 = time.time()
+    rhs_body = fin.FD_outputC("returnstring", evol_rhss, params="outCverbose=False", upwindcontrolvec=betaU)
+    print("(BENCH) rhs_eval codegen %.1fs" % (time.time() - t0), flush=True)
+    rhs_params = ("rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
+                  "const REAL *restrict auxevol_gfs,const REAL *restrict in_gfs,REAL *restrict rhs_gfs")
+    rhs_src = oc.outCfunction(outfile="returnstring", desc="Evaluate the BSSN RHSs", name="rhs_eval",
+                              params=rhs_params, body=rhs_body, loopopts="InteriorPoints,Enable_rfm_precompute")
+    args_decl = ("const paramstruct params, const rfm_struct rfmstruct, const double *__restrict__ auxevol_gfs, "
+                 "const double *__restrict__ in_gfs")
+    fused = bd.fused_rk4_kernel("rhs_eval_rk4_substep", args_decl, "", rhs_body,
+                                loopopts="InteriorPoints,Enable_rfm_precompute")
+    fused += _fused_launcher("rhs_eval_rk4_substep", "const rfm_struct *rfmstruct, const paramstruct *params, "
+                             "const double *auxevol_gfs, const double *in_gfs",
+                             "*params, *rfmstruct, auxevol_gfs, in_gfs", "InteriorPoints")
+    t0 = time.time()
+    ricci_body = fin.FD_outputC("returnstring",
+                                [lhrh(lhs=gri.gfaccess("auxevol_gfs", "RbarDD%d%d" % (i, j)), rhs=RbarDD[i][j])
+                                 for i in range(3) for j in range(i, 3)], params="outCverbose=False")
+    print("(BENCH) Ricci codegen %.1fs" % (time.time() - t0), flush=True)
+    ricci_src = oc.outCfunction(outfile="returnstring", desc="Evaluate the Ricci tensor", name="Ricci_eval",
+                                params="rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
+                                       "const REAL *restrict in_gfs,REAL *restrict auxevol_gfs",
+                                body=ricci_body, loopopts="InteriorPoints,Enable_rfm_precompute")
+    t0 = time.time()
+    h_body = fin.FD_outputC("returnstring", lhrh(lhs=gri.gfaccess("aux_gfs", "H"), rhs=bssncon.H),
+                            params="outCverbose=False")
+    print("(BENCH) Hamiltonian codegen %.1fs" % (time.time() - t0), flush=True)
+    h_src = oc.outCfunction(outfile="returnstring", desc="Evaluate the Hamiltonian constraint",
+                            name="Hamiltonian_constraint",
+                            params="rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
+                                   "const REAL *restrict in_gfs, REAL *restrict aux_gfs",
+                            body=h_body, loopopts="InteriorPoints,Enable_rfm_precompute")
+    detg_body = fin.FD_outputC("returnstring", detg_exprs, params="outCverbose=False")
+    detg_src = oc.outCfunction(outfile="returnstring", desc="Enforce det(gammabar) = det(gammahat) constraint.",
+                               name="enforce_detgammabar_constraint",
+                               params="const rfm_struct *restrict rfmstruct,const paramstruct *restrict params, "
+                                      "REAL *restrict in_gfs",
+                               body=detg_body, loopopts="AllPoints,Enable_rfm_precompute")
+    # Precompute tables depend on which functions remain; build the struct now.
+    rfm_code = bd.rfm_struct_code()
+    tables = [(n, str(e), list(a)) for n, e, a in rfm.rfm_precompute_tables()]
+    par.set_parval_from_str("reference_metric::enable_rfm_precompute", "False")
+    rfm.ref_metric__hatted_quantities()
+    t0 = time.time()
+    id_src = _initial_data_kernel(id_rhss, id_names)
+    print("(BENCH) initial data codegen %.1fs" % (time.time() - t0), flush=True)
+    pre, meta = _header(FD_order // 2 + 1)
+    parts = [pre, bd.coord_host_functions(CoordSystem), '#include "../nrpy_runtime.h"', rfm_code,
+             ricci_src, rhs_src, fused, h_src, detg_src, id_src,
+             _mol_glue(True, True, None, "rfmstruct, params, auxevol_gfs, IN")]
+    meta.update({"bundle": bundle_name, "CoordSystem": CoordSystem, "symmetry_axes": symmetry_axes,
+                 "rfm_tables": tables, "has_ricci": True, "has_detg": True,
+                 "kernels": ["Ricci_eval", "rhs_eval", "Hamiltonian_constraint", "enforce_detgammabar_constraint",
+                             "initial_data"]})
+    return bd.write_bundle(bundle_name, parts, meta)
+
+
+def generate(name):
+    if name == "scalarwave_cartesian":
+        return scalarwave_cartesian()
+    if name == "bssn_cartesian":
+        return bssn("Cartesian", "", name)
+    if name == "bssn_spherical_sym2":
+        return bssn("Spherical", "2", name)
+    if name == "maxwell_cartesian":
+        from . import maxwell
+        return maxwell.maxwell_cartesian()
+    raise ValueError("unknown bundle " + name)
+
+
+if __name__ == "__main__":
+    for n in sys.argv[1:]:
+        t = time.time()
+        print("generated", generate(n), "in %.1fs" % (time.time() - t), flush=True)

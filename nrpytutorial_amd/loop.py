@@ -1,0 +1,176 @@
+"""Loop generation -> gfx950 launch geometry.
+
+``simple_loop(options, interior)`` keeps the option vocabulary of loop.py
+(loop.py:129-194): ``AllPoints`` / ``InteriorPoints`` choose the index range
+([0, N+2G) or [G, N+G) in each direction), ``Read_xxs`` makes the coordinates
+xx0, xx1, xx2 available, ``Enable_rfm_precompute`` reads the precomputed
+reference-metric tables, and ``EnableSIMD`` / ``DisableOpenMP`` /
+``OMP_custom_pragma`` are accepted and have no effect.
+
+Instead of a triple loop, the body runs once per lane: i0 (the fastest memory
+direction) maps to threadIdx.x so a 64-lane wavefront reads 64 consecutive
+doubles of every gridfunction, i1 to threadIdx.y and i2 to blockIdx.z.
+``loop1D`` / ``loop`` produce plain C loops for host-side code.
+"""
+import re
+from . import NRPy_param_funcs as par
+
+BLOCK_X = 64
+BLOCK_Y = 4
+
+
+def loop1D(idx_var="i", lower_bound="0", upper_bound="N", increment="1", pragma="#pragma omp parallel for"):
+    header = "%sfor (int %s = %s; %s < %s; %s += %s) {" % (
+        (pragma + "\n") if pragma else "", idx_var, lower_bound, idx_var, upper_bound, idx_var, increment)
+    return header, "} // END LOOP: for (int %s = %s; %s < %s; %s += %s)" % (
+        idx_var, lower_bound, idx_var, upper_bound, idx_var, increment)
+
+
+def loop(idx_var, lower_bound, upper_bound, increment, pragma, padding="  ", interior="", tile_size=""):
+    if not isinstance(idx_var, list):
+        idx_var, lower_bound, upper_bound, increment, pragma = \
+            [idx_var], [lower_bound], [upper_bound], [increment], [pragma]
+    heads, tails = [], []
+    for i in range(len(idx_var)):
+        h, t = loop1D(idx_var[i], lower_bound[i], upper_bound[i], increment[i], pragma[i])
+        heads.append(padding * i + h)
+        tails.append(padding * i + t)
+    body = "\n".join(padding * len(idx_var) + l for l in interior.splitlines())
+    return "\n".join(heads) + ("\n" + body if body else "") + "\n" + "\n".join(reversed(tails)) + "\n"
+
+
+def parse_loopopts(options):
+    opts = {"range": None, "Read_xxs": False, "rfm": False}
+    for o in [s.strip() for s in options.split(",") if s.strip()]:
+        if o == "AllPoints":
+            opts["range"] = "all"
+        elif o == "InteriorPoints":
+            opts["range"] = "interior"
+        elif o == "Read_xxs":
+            opts["Read_xxs"] = True
+        elif o == "Enable_rfm_precompute":
+            opts["rfm"] = True
+        elif o in ("EnableSIMD", "DisableOpenMP", "oldloops") or o.startswith("OMP_custom_pragma") \
+                or o.startswith("OMP_collapse"):
+            pass
+        else:
+            raise ValueError("unknown loop option '" + o + "'")
+    if opts["range"] is None:
+        raise ValueError("loop options must contain AllPoints or InteriorPoints")
+    if opts["Read_xxs"] and opts["rfm"]:
+        raise ValueError("Read_xxs and Enable_rfm_precompute are mutually exclusive")
+    return opts
+
+
+def rfm_reads(body):
+    """Statements reading precomputed reference-metric tables used in ``body``."""
+    from . import reference_metric as rfm
+    out = []
+    for name, expr, axes in rfm.rfm_precompute_tables():
+        if not re.search(r"\b%s\b" % re.escape(name), body):
+            continue
+        if len(axes) == 1:
+            out.append("const double %s = rfmstruct.%s[i%d];" % (name, name, axes[0]))
+        elif axes == (0, 1):
+            out.append("const double %s = rfmstruct.%s[i0 + Nxx_plus_2NGHOSTS0*i1];" % (name, name))
+        else:
+            raise ValueError("unsupported reference-metric table dependence for " + name)
+    return out
+
+
+def simple_loop(options, interior):
+    """Kernel body: lane -> gridpoint mapping, bounds check, coordinate/table reads, ``interior``."""
+    o = parse_loopopts(options)
+    lo = "NGHOSTS" if o["range"] == "interior" else "0"
+    s = ""
+    for d, src in ((0, "threadIdx.x + blockIdx.x*blockDim.x"), (1, "threadIdx.y + blockIdx.y*blockDim.y"),
+                   (2, "blockIdx.z")):
+        s += "const int i%d = %s + (%s);\n" % (d, lo, src)
+    if o["range"] == "interior":
+        s += "if (i0 >= Nxx0 + NGHOSTS || i1 >= Nxx1 + NGHOSTS || i2 >= Nxx2 + NGHOSTS) return;\n"
+    else:
+        s += ("if (i0 >= Nxx_plus_2NGHOSTS0 || i1 >= Nxx_plus_2NGHOSTS1 || i2 >= Nxx_plus_2NGHOSTS2) return;\n")
+    if o["Read_xxs"]:
+        for d in range(3):
+            if re.search(r"\bxx%d\b" % d, interior):
+                s += "const double xx%d = xx%d_arr[i%d];\n" % (d, d, d)
+    if o["rfm"]:
+        s += "\n".join(rfm_reads(interior)) + "\n"
+    return s + interior
+
+
+def _used_cparams(text):
+    names = set(re.findall(r"\b[A-Za-z_][A-Za-z_0-9]*\b", text))
+    out = []
+    for name, ctype in par.paramstruct_members():
+        if name in ("xx0", "xx1", "xx2"):
+            continue
+        if name in names:
+            out.append((name, ctype))
+    return out
+
+
+def _ctype(c):
+    return {"REAL": "double", "int": "int", "bool": "int", "char": "const char *"}.get(c, c)
+
+
+def kernel_and_launcher(name, desc, args, preloop, body, loopopts, postloop):
+    """Assemble a ``__global__`` kernel and its ``extern "C"`` launcher.
+
+    ``args`` is the reference-style parameter list [(ctype, name)]:
+    ``paramstruct *params`` and ``rfm_struct *rfmstruct`` become by-value
+    kernel arguments, ``REAL *xx[3]`` becomes three coordinate arrays and every
+    other pointer is a device array. The launcher appends ``hipStream_t stream``
+    and returns 0 on success or a negative error code."""
+    o = parse_loopopts(loopopts)
+    kbody = simple_loop(loopopts, body)
+    full = preloop + "\n" + kbody + "\n" + postloop
+    kargs, largs, call = [], [], []
+    for ctype, an in args:
+        base = ctype.replace("const", "").replace("*", "").strip()
+        if base == "paramstruct":
+            kargs.append("const paramstruct params")
+            largs.append("const paramstruct *params")
+            call.append("*params")
+        elif base == "rfm_struct":
+            kargs.append("const rfm_struct rfmstruct")
+            largs.append("const rfm_struct *rfmstruct")
+            call.append("*rfmstruct")
+        elif an == "xx" and "[3]" in ctype:
+            kargs += ["const double *__restrict__ xx%d_arr" % d for d in range(3)]
+            largs.append("double *xx[3]")
+            call += ["xx[%d]" % d for d in range(3)]
+        else:
+            isconst = "const" in ctype
+            kargs.append(("const double *__restrict__ " if isconst else "double *__restrict__ ") + an)
+            largs.append(("const double *" if isconst else "double *") + an)
+            call.append(an)
+    needs = _used_cparams(full)
+    unpack = "".join("  const %s %s = params.%s;\n" % (_ctype(t), n, n) for n, t in needs)
+    # always needed for bounds and indexing
+    for n in ("Nxx0", "Nxx1", "Nxx2", "Nxx_plus_2NGHOSTS0", "Nxx_plus_2NGHOSTS1", "Nxx_plus_2NGHOSTS2"):
+        if not any(n == x for x, _ in needs):
+            unpack += "  const int %s = params.%s;\n" % (n, n)
+    kname = name + "__gpu"
+    kernel = ("/* %s */\n__global__ void __launch_bounds__(%d) %s(%s) {\n%s%s}\n" %
+              (desc, BLOCK_X * BLOCK_Y, kname, ", ".join(kargs), unpack,
+               "".join("  " + l + "\n" for l in full.splitlines())))
+    rng = "interior" if o["range"] == "interior" else "all"
+    if rng == "interior":
+        n0, n1, n2 = "params->Nxx0", "params->Nxx1", "params->Nxx2"
+    else:
+        n0, n1, n2 = "params->Nxx_plus_2NGHOSTS0", "params->Nxx_plus_2NGHOSTS1", "params->Nxx_plus_2NGHOSTS2"
+    ptr_checks = " || ".join("!" + c for c in call if not c.startswith("*") and "xx[" not in c) or "0"
+    launcher = (
+        'extern "C" int %s(%s, hipStream_t stream) {\n'
+        "  if (!params) return NRPY_ERR_NULL;\n"
+        "  if (%s) return NRPY_ERR_NULL;\n"
+        "  if (nrpy_check_params(params)) return NRPY_ERR_SHAPE;\n"
+        "  const dim3 block(%d, %d, 1);\n"
+        "  const dim3 grid((%s + %d - 1) / %d, (%s + %d - 1) / %d, %s);\n"
+        "  hipLaunchKernelGGL(%s, grid, block, 0, stream, %s);\n"
+        "  return hipGetLastError() == hipSuccess ? 0 : NRPY_ERR_LAUNCH;\n"
+        "}\n" % (name, ", ".join(largs), ptr_checks, BLOCK_X, BLOCK_Y, n0, BLOCK_X, BLOCK_X, n1, BLOCK_Y, BLOCK_Y,
+                 n2, kname, ", ".join(call)))
+    proto = 'extern "C" int %s(%s, hipStream_t stream);' % (name, ", ".join(largs))
+    return kernel, launcher, proto
