@@ -1,0 +1,231 @@
+"""Benchmark: FP64 gridpoint-updates/s of the BSSN RK4 method-of-lines step.
+
+Workload (plain BSSN vacuum forward): BSSN in the Cartesian reference metric
+on a 192^3 interior grid (NGHOSTS=3, [-10,10]^3), 24 evolved FP64
+gridfunctions, 4th-order FD with upwinded shift advection, 1+log lapse,
+covariant Gamma-driver shift (eta=2), RK4 with dt = 0.5*dx; the state is the
+flat metric plus 8 seeded Gaussians (amplitude 1e-3, width 1-3) in every
+field. One step = one RK4 step = per substep Ricci_eval, the fused RHS + RK
+update kernel, the ghost-zone fill and det(gammabar) enforcement, replayed
+from a hipGraph. A gridpoint-update is one RHS evaluation at one interior
+point (4 per RK4 step), the reference's gp/s.
+
+Also reported: the standalone RHS throughput on the same state, the
+Brill-Lindquist head-on evolution on Spherical 72x12x2 (50 RK4 steps, the
+latency-bound small case), per-kernel HIP-event timings and the roofline of
+the dominant kernel, and the CPU baseline: the C oracle built with the
+reference's optimized flags, OpenMP over the host cores, on a bounded sample.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 192] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+METRIC = "FP64 gridpoint-updates/s: BSSN RK4 forward, adjoint, batched vs plain GPU"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline_worker(n, min_seconds, max_steps):
+    """Runs in a child process (fresh OpenMP runtime): oracle RK4 steps on an n^3 grid."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    from oracle_lib import Oracle, OracleGrid
+    from nrpytutorial_amd.workloads import bssn_gaussian_state_numpy
+    og = OracleGrid(Oracle("bssn_cartesian", fast=True), (n, n, n), {})
+    y0 = bssn_gaussian_state_numpy(og.xx, og.o.evol, 1)
+    og.apply_bcs(y0)
+    og.enforce_detg(y0)
+    dt = 0.5 * og.p.dxx0
+    y = y0.copy()
+    og.mol_step(y, dt, 1)  # warm-up
+    steps, el = 0, 0.0
+    # The synthetic data are re-seeded every 10 steps (outside the timer) so the
+    # sample stays a short evolution of the benchmark state.
+    while steps < max_steps and el < min_seconds:
+        y[...] = y0
+        t0 = time.perf_counter()
+        og.mol_step(y, dt, 10)
+        el += time.perf_counter() - t0
+        steps += 10
+        if not np.isfinite(y).all():
+            raise RuntimeError("oracle state became non-finite")
+    return {"gp_per_s": 4.0 * n ** 3 * steps / el, "steps": steps, "seconds": el}
+
+
+def cpu_baseline(n=64, min_seconds=12.0, max_steps=200):
+    cores = int(os.environ.get("NRPY_CPU_BASELINE_THREADS", min(16, os.cpu_count() or 1)))
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", str(n),
+                        str(min_seconds), str(max_steps)], env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline failed: " + r.stderr[-2000:])
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": res["gp_per_s"], "unit": "gridpoint-updates/s", "cores": cores, "kind": "port",
+            "sample": "C oracle (gcc -Ofast -fopenmp -march=native -funroll-loops), BSSN Cartesian %d^3 seeded "
+                      "Gaussians, %d RK4 steps in %.1f s, OMP_NUM_THREADS=%d, CPU: %s"
+                      % (n, res["steps"], res["seconds"], cores, model)}
+
+
+def time_kernel(torch, fn, reps):
+    """Average duration (s) of fn() launched on torch's current stream, HIP events."""
+    fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    e.synchronize()
+    return s.elapsed_time(e) / 1000.0 / reps
+
+
+def bench_bl(torch):
+    """Brill-Lindquist head-on, Spherical 72x12x2, RMAX=7.5, CFL 0.5, 50 RK4 steps."""
+    from nrpytutorial_amd.runtime import Bundle, Grid, Evolution
+    b = Bundle("bssn_spherical_sym2")
+    pars = dict(BH1_mass=0.5, BH2_mass=0.5, BH1_posn_z=0.5, BH2_posn_z=-0.5, RMAX=7.5, eta=2.0)
+    g = Grid(b, (72, 12, 2), **pars)
+    ev = Evolution(g)
+    ev.initial_data()
+    dt = g.find_timestep(0.5)
+    ev.step(dt, 5)
+    torch.cuda.synchronize()
+    ev.initial_data()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev.step(dt, 50)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    H = g.interior(ev.hamiltonian())
+    l2 = float(torch.sqrt(torch.mean(H * H)))
+    return {"gp_per_s": 4.0 * 72 * 12 * 2 * 50 / el, "seconds": el, "H_L2": l2, "dt": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=192)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-baseline-worker", nargs=3)
+    a = ap.parse_args()
+    if a.cpu_baseline_worker:
+        n, sec, mx = a.cpu_baseline_worker
+        print(json.dumps(cpu_baseline_worker(int(n), float(sec), int(mx))))
+        return 0
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        return 2
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 1
+    from nrpytutorial_amd.runtime import Bundle, Grid, Evolution, ptr, check
+    from nrpytutorial_amd.workloads import bssn_gaussian_state_torch
+
+    N = a.n
+    b = Bundle("bssn_cartesian")
+    g = Grid(b, (N, N, N))
+    ev = Evolution(g)
+    bssn_gaussian_state_torch(ev.y_n, g.xx_dev, b.evol, seed=1)
+    ev.apply_bcs(ev.y_n)
+    ev.post_bcs(ev.y_n)
+    dt = 0.5 * g.params.dxx0
+    torch.cuda.synchronize()
+    log("grid %d^3 (%d points incl. ghosts), dt=%g" % (N, g.Ntot, dt))
+
+    ev.step(dt, a.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev.step(dt, a.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    gp = 4.0 * N ** 3 * a.steps / el
+    if not torch.isfinite(ev.y_n).all():
+        raise RuntimeError("non-finite state after the timed steps")
+    log("RK4: %d steps in %.4f s -> %.4e gp/s, %.3f ms/step" % (a.steps, el, gp, 1e3 * el / a.steps))
+
+    # Per-kernel timings (HIP events on torch's current stream = the launch stream).
+    s = ev.s()
+    reps = 10
+    cost = b.meta["cost"]
+    npts = N ** 3
+    t_ricci = time_kernel(torch, lambda: check(b.lib.Ricci_eval(g.rfm, g.pp(), ptr(ev.y_n), ptr(ev.auxevol), s),
+                                               "Ricci"), reps)
+    t_fused = time_kernel(torch, lambda: check(b.lib.rhs_eval_rk4_substep(
+        1, g.rfm, g.pp(), ptr(ev.auxevol), ptr(ev.k_odd), ptr(ev.y_n), ptr(ev.rt), ptr(ev.k_even), dt, s),
+        "rhs_eval_rk4_substep"), reps)
+    rhs_tmp = ev.k_even
+    t_rhs = time_kernel(torch, lambda: check(b.lib.rhs_eval(g.rfm, g.pp(), ptr(ev.auxevol), ptr(ev.y_n),
+                                                            ptr(rhs_tmp), s), "rhs_eval"), reps)
+    t_bc = time_kernel(torch, lambda: ev.apply_bcs(ev.k_odd, composed=True), reps)
+    t_detg = time_kernel(torch, lambda: ev.post_bcs(ev.k_odd), reps)
+    ngf = len(b.evol)
+    fused_flops = cost["rhs_eval"]["flops"] + 6 * ngf
+    fused_bytes = 8 * (ngf + 6 + ngf + ngf + ngf + ngf)  # read X, Rbar, y_n, rt; write out, rt (substep 1)
+    achieved_tf = fused_flops * npts / t_fused / 1e12
+    achieved_gbs = fused_bytes * npts / t_fused / 1e9
+    rhs_only_gp = npts / (t_ricci + t_rhs)
+    log("kernel avg: Ricci %.3f ms (%.1f TF/s), fused RHS+RK %.3f ms (%.1f TF/s, %.0f GB/s), rhs_eval %.3f ms, "
+        "ghost fill %.3f ms, detg %.3f ms" % (1e3 * t_ricci, cost["Ricci_eval"]["flops"] * npts / t_ricci / 1e12,
+                                              1e3 * t_fused, achieved_tf, achieved_gbs, 1e3 * t_rhs, 1e3 * t_bc,
+                                              1e3 * t_detg))
+    bl = bench_bl(torch)
+    log("BL 72x12x2: 50 steps in %.4f s -> %.3e gp/s, H-L2 %.6e" % (bl["seconds"], bl["gp_per_s"], bl["H_L2"]))
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline()
+            log("CPU baseline: %.3e gp/s (%s)" % (cpu["value"], cpu["sample"]))
+        except Exception as e:  # reported, never replaces the GPU number
+            log("CPU baseline failed: %s" % e)
+    out = {
+        "metric": METRIC, "value": gp, "unit": "gridpoint-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * el / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded flat + 8 Gaussians per gridfunction)",
+        "config": {"workload": "BSSN vacuum RK4 step, Cartesian reference metric, %d^3 interior, FD4 + upwinding, "
+                               "1+log lapse, covariant Gamma-driver (eta=2), hipGraph-replayed substeps" % N,
+                   "grid": [N, N, N], "NGHOSTS": 3, "gridfunctions": ngf, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "rhs_eval_rk4_substep (FP64 VALU; no matrix cores)",
+                     "flops_per_point": fused_flops, "bytes_per_point": fused_bytes,
+                     "avg_launch_ms": 1e3 * t_fused, "hbm_GBps_algorithmic": achieved_gbs},
+        "cpu_baseline": cpu,
+        "extra": {"rhs_only_gp_per_s": rhs_only_gp, "kernel_ms": {"Ricci_eval": 1e3 * t_ricci,
+                                                                  "rhs_eval_rk4_substep": 1e3 * t_fused,
+                                                                  "rhs_eval": 1e3 * t_rhs,
+                                                                  "ghost_fill": 1e3 * t_bc, "detg": 1e3 * t_detg},
+                  "ricci_flops_per_point": cost["Ricci_eval"]["flops"],
+                  "bl_72x12x2_50steps": bl},
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -261,6 +261,55 @@ def fused_rk4_kernel(name, args_decl, call_args, body_with_writes, loopopts="Int
             (lp.BLOCK_X * lp.BLOCK_Y, name, args_decl, unpack, "".join("  " + l + "\n" for l in kbody.splitlines())))
 
 
+_TOKEN = re.compile(r"[A-Za-z_][A-Za-z_0-9]*|\d+\.\d*(?:[eE][-+]?\d+)?|\d+|\S")
+_FUNCS = ("sqrt", "cbrt", "pow", "exp", "log", "sin", "cos", "tan", "fabs", "sinh", "cosh", "tanh", "asinh",
+          "acos", "atan2")
+
+
+def count_body(body):
+    """Algorithmic cost of one gridpoint of a generated body.
+
+    FLOPs: every binary +, -, *, / and every math-library call counts 1, so a
+    fused multiply-add counts 2. Bytes: 8 per distinct (array, gridfunction)
+    read and per distinct gridfunction written, i.e. each input read once and
+    each output written once (stencil re-reads are not counted)."""
+    flops = 0
+    reads, writes = set(), set()
+    for line in body.splitlines():
+        line = line.strip()
+        if "=" not in line or line.startswith("/*") or line.startswith("//"):
+            continue
+        lhs, rhs = line.split("=", 1)
+        m = re.search(r"(\w+)\[IDX4\((\w+)GF", lhs)
+        if m:
+            writes.add((m.group(1), m.group(2)))
+        for a, g in re.findall(r"(\w+)\[IDX4\((\w+)GF", rhs):
+            reads.add((a, g))
+        if "[IDX4(" in rhs and re.match(r"\s*\w+\[IDX4\([^)]*\)\];\s*$", rhs):
+            continue  # plain load
+        toks = _TOKEN.findall(rhs.rstrip(";"))
+        prev_operand = False
+        for t in toks:
+            if t in ("+", "-"):
+                if prev_operand:
+                    flops += 1
+                prev_operand = False
+            elif t in ("*", "/"):
+                flops += 1
+                prev_operand = False
+            elif t in _FUNCS:
+                flops += 1
+                prev_operand = False
+            elif t == "(" or t == ",":
+                prev_operand = False
+            elif t in ("?", ":", ">", "<"):
+                prev_operand = False
+            else:
+                prev_operand = True
+    return {"flops": flops, "read_gfs": len(reads), "write_gfs": len(writes),
+            "bytes": 8 * (len(reads) + len(writes))}
+
+
 def write_bundle(bundle_name, parts, meta):
     """Write csrc/generated/<bundle>.hip and <bundle>.json."""
     os.makedirs(GENDIR, exist_ok=True)

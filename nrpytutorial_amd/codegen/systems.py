@@ -27,7 +27,7 @@ from .. import outputC as oc
 from ..outputC import lhrh
 from . import bundle as bd
 
-BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2", "maxwell_cartesian")
+BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2")
 
 
 def _mol_glue(has_ricci, has_detg, rhs_args_decl, rhs_call_args):
@@ -157,6 +157,7 @@ def scalarwave_cartesian():
     parts = [pre, bd.coord_host_functions("Cartesian"), '#include "../nrpy_runtime.h"', bd.rfm_struct_code(),
              rhs_src, fused, id_src,
              _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN")]
+    meta["cost"] = {"rhs_eval": bd.count_body(body)}
     meta.update({"bundle": "scalarwave_cartesian", "CoordSystem": "Cartesian", "rfm_tables": [],
                  "kernels": ["rhs_eval", "initial_data"], "has_ricci": False, "has_detg": False})
     return bd.write_bundle("scalarwave_cartesian", parts, meta)
@@ -255,6 +256,9 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     parts = [pre, bd.coord_host_functions(CoordSystem), '#include "../nrpy_runtime.h"', rfm_code,
              ricci_src, rhs_src, fused, h_src, detg_src, id_src,
              _mol_glue(True, True, None, "rfmstruct, params, auxevol_gfs, IN")]
+    meta["cost"] = {"rhs_eval": bd.count_body(rhs_body), "Ricci_eval": bd.count_body(ricci_body),
+                    "Hamiltonian_constraint": bd.count_body(h_body),
+                    "enforce_detgammabar_constraint": bd.count_body(detg_body)}
     meta.update({"bundle": bundle_name, "CoordSystem": CoordSystem, "symmetry_axes": symmetry_axes,
                  "rfm_tables": tables, "has_ricci": True, "has_detg": True,
                  "kernels": ["Ricci_eval", "rhs_eval", "Hamiltonian_constraint", "enforce_detgammabar_constraint",

@@ -20,6 +20,12 @@ from helpers import rel_maxnorm_err
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
 TOL_EVOL = 1e-10
+# Brill-Lindquist data are conformally flat: h_ij and lambda^i vanish analytically and
+# hold only cancellation noise (~1e-14 absolute) at t=0, and stay far below the O(1)
+# fields during short evolutions. Gridfunctions smaller than FLOOR_* times the state's
+# largest max-norm are therefore measured against that fraction of the state's scale.
+FLOOR_ID = 1e-1
+FLOOR_EVOL = 1e-3
 
 
 def _cart(torch, Nxx, seed=7):
@@ -97,13 +103,13 @@ def test_spherical_bl_initial_data_bcs(torch_cuda):
     y_ref = og.zeros(len(b.evol))
     og.initial_data(y_ref)
     # ghost zones of the closed-form data include r<0 and theta<0 points: compare the interior
-    err0 = rel_maxnorm_err(g.interior(ev.y_n.cpu().numpy()), og.interior(y_ref))
+    err0 = rel_maxnorm_err(g.interior(ev.y_n.cpu().numpy()), og.interior(y_ref), floor_frac=FLOOR_ID)
     ev.apply_bcs(ev.y_n)
     ev.post_bcs(ev.y_n)
     og.apply_bcs(y_ref)
     og.enforce_detg(y_ref)
     torch.cuda.synchronize()
-    err1 = rel_maxnorm_err(ev.y_n.cpu().numpy(), y_ref)
+    err1 = rel_maxnorm_err(ev.y_n.cpu().numpy(), y_ref, floor_frac=FLOOR_ID, report=True)
     print("ID interior %.2e, after BCs+detg all points %.2e" % (err0, err1))
     assert err0 < TOL and err1 < TOL
 
@@ -124,7 +130,7 @@ def test_spherical_bl_evolution(torch_cuda, Nxx, nsteps):
     og.mol_step(y_ref, dt, nsteps)
     H = ev.hamiltonian()
     torch.cuda.synchronize()
-    err = rel_maxnorm_err(ev.y_n.cpu().numpy(), y_ref)
+    err = rel_maxnorm_err(ev.y_n.cpu().numpy(), y_ref, floor_frac=FLOOR_EVOL, report=True)
     H_ref = og.interior(og.hamiltonian(y_ref))
     Hg = g.interior(H.cpu().numpy())
     l2, l2_ref = np.sqrt(np.mean(Hg ** 2)), np.sqrt(np.mean(H_ref ** 2))
