@@ -248,7 +248,22 @@ WRITE_RE = re.compile(r"rhs_gfs\[IDX4\((\w+)GF, i0, i1, i2\)\] = (.*);")
 
 def fused_rk4_kernel(name, args_decl, call_args, body_with_writes, loopopts="InteriorPoints,Enable_rfm_precompute"):
     """Template kernel evaluating the RHS and applying RK4 substep S at each interior point."""
-    fused_body = WRITE_RE.sub(lambda m: "NRPY_RK4_UPDATE(S, %sGF, %s);" % (m.group(1), m.group(2)), body_with_writes)
+    # The RHS values are computed first; the RK update (loads of y_n and the
+    # running total) comes after a scheduling barrier so the compiler does not
+    # hoist those loads to the top of the kernel, where they would stay live
+    # through the whole RHS evaluation.
+    lines = body_with_writes.splitlines()
+    vals = []
+    for i, l in enumerate(lines):
+        m = WRITE_RE.search(l)
+        if m:
+            lines[i] = "const double __rhs_%s = %s;" % (m.group(1), m.group(2))
+            vals.append(m.group(1))
+    tail = ["__builtin_amdgcn_sched_barrier(0);"] + ["NRPY_RK4_UPDATE(S, %sGF, __rhs_%s);" % (v, v) for v in vals]
+    # insert inside the body's closing brace, where the __rhs_* values are in scope
+    close = max(i for i, l in enumerate(lines) if l.strip() == "}")
+    lines[close:close] = tail
+    fused_body = "\n".join(lines)
     if "rhs_gfs" in fused_body:
         raise ValueError("unrecognised write to rhs_gfs in the RHS body")
     kbody = lp.simple_loop(loopopts, fused_body)
@@ -258,7 +273,7 @@ def fused_rk4_kernel(name, args_decl, call_args, body_with_writes, loopopts="Int
             unpack += "  const int %s = params.%s;\n" % (n, n)
     return ("template <int S>\n__global__ void __launch_bounds__(%d) %s__gpu(%s, double *rk_yn, double *rk_rt, "
             "double *rk_out, const double dt) {\n%s%s}\n" %
-            (lp.BLOCK_X * lp.BLOCK_Y, name, args_decl, unpack, "".join("  " + l + "\n" for l in kbody.splitlines())))
+            (lp.MAX_THREADS, name, args_decl, unpack, "".join("  " + l + "\n" for l in kbody.splitlines())))
 
 
 _TOKEN = re.compile(r"[A-Za-z_][A-Za-z_0-9]*|\d+\.\d*(?:[eE][-+]?\d+)?|\d+|\S")

@@ -93,7 +93,7 @@ def _fused_launcher(name, args_sig, call_args, loopopts):
     rng_interior = "InteriorPoints" in loopopts
     n = ("params->Nxx0", "params->Nxx1", "params->Nxx2") if rng_interior else \
         ("params->Nxx_plus_2NGHOSTS0", "params->Nxx_plus_2NGHOSTS1", "params->Nxx_plus_2NGHOSTS2")
-    bx, by = bd.lp.BLOCK_X, bd.lp.BLOCK_Y
+    kid = bd.lp.kernel_id(name)
     cases = "".join(
         "    case %d: hipLaunchKernelGGL(%s__gpu<%d>, grid, block, 0, stream, %s, y_n, rt, out, dt); break;\n"
         % (s, name, s, call_args) for s in range(4))
@@ -102,11 +102,34 @@ def _fused_launcher(name, args_sig, call_args, loopopts):
             "  if (!params || !y_n || !rt || !out) return NRPY_ERR_NULL;\n"
             "  if (S < 0 || S > 3) return NRPY_ERR_PARAM;\n"
             "  if (nrpy_check_params(params)) return NRPY_ERR_SHAPE;\n"
-            "  const dim3 block(%d, %d, 1);\n"
-            "  const dim3 grid((%s + %d - 1) / %d, (%s + %d - 1) / %d, %s);\n"
+            "  const dim3 block = nrpy_block(%d, %d, %d, %d);\n"
+            "  const dim3 grid(nrpy_nblocks(block, %s, %s, %s));\n"
             "  switch (S) {\n%s  }\n"
             "  return hipGetLastError() == hipSuccess ? 0 : NRPY_ERR_LAUNCH;\n}\n"
-            % (name, args_sig, bx, by, n[0], bx, bx, n[1], by, by, n[2], cases))
+            % (name, args_sig, kid, bd.lp.BLOCK_X, bd.lp.BLOCK_Y, bd.lp.BLOCK_Z, n[0], n[1], n[2], cases))
+
+
+# Output groups of the BSSN RHS, one kernel each (fewer live values per kernel).
+DEFAULT_RHS_GROUPS = "aDD,hDD,alpha,cf,trK|lambdaU,vetU,betU"
+
+
+def rhs_groups(names):
+    """Partition RHS output names by prefix groups from NRPY_BSSN_RHS_GROUPS
+    (``|``-separated groups of comma-separated name prefixes; ``all`` = one kernel)."""
+    import os
+    spec = os.environ.get("NRPY_BSSN_RHS_GROUPS", DEFAULT_RHS_GROUPS)
+    if spec == "all":
+        return [list(names)]
+    groups = []
+    for g in spec.split("|"):
+        prefixes = [x for x in g.split(",") if x]
+        sel = [n for n in names if any(n.startswith(p) for p in prefixes)]
+        if sel:
+            groups.append(sel)
+    covered = [n for g in groups for n in g]
+    if sorted(covered) != sorted(names):
+        raise ValueError("RHS groups do not partition the outputs: " + spec)
+    return groups
 
 
 def _header(NGHOSTS):
@@ -207,19 +230,38 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     pairs = sorted(zip(lhs_names, rhs_exprs), key=lambda p: p[0])
     evol_rhss = [lhrh(lhs=gri.gfaccess("rhs_gfs", n), rhs=e) for n, e in pairs]
     t0 = time.time()
-    rhs_body = fin.FD_outputC("returnstring", evol_rhss, params="outCverbose=False", upwindcontrolvec=betaU)
-    print("(BENCH) rhs_eval codegen %.1fs" % (time.time() - t0), flush=True)
+    groups = rhs_groups([n for n, _ in pairs])
     rhs_params = ("rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
                   "const REAL *restrict auxevol_gfs,const REAL *restrict in_gfs,REAL *restrict rhs_gfs")
-    rhs_src = oc.outCfunction(outfile="returnstring", desc="Evaluate the BSSN RHSs", name="rhs_eval",
-                              params=rhs_params, body=rhs_body, loopopts="InteriorPoints,Enable_rfm_precompute")
     args_decl = ("const paramstruct params, const rfm_struct rfmstruct, const double *__restrict__ auxevol_gfs, "
                  "const double *__restrict__ in_gfs")
-    fused = bd.fused_rk4_kernel("rhs_eval_rk4_substep", args_decl, "", rhs_body,
-                                loopopts="InteriorPoints,Enable_rfm_precompute")
-    fused += _fused_launcher("rhs_eval_rk4_substep", "const rfm_struct *rfmstruct, const paramstruct *params, "
-                             "const double *auxevol_gfs, const double *in_gfs",
-                             "*params, *rfmstruct, auxevol_gfs, in_gfs", "InteriorPoints")
+    rhs_src, fused, group_costs = "", "", []
+    for k, grp in enumerate(groups):
+        sel = [e for (n, _), e in zip(pairs, evol_rhss) if n in grp]
+        body = fin.FD_outputC("returnstring", sel, params="outCverbose=False", upwindcontrolvec=betaU)
+        group_costs.append(bd.count_body(body))
+        rhs_src += oc.outCfunction(outfile="returnstring", desc="BSSN RHSs of %s" % ", ".join(grp),
+                                   name="rhs_eval_g%d" % k, params=rhs_params, body=body,
+                                   loopopts="InteriorPoints,Enable_rfm_precompute")
+        fused += bd.fused_rk4_kernel("rhs_eval_rk4_substep_g%d" % k, args_decl, "", body,
+                                     loopopts="InteriorPoints,Enable_rfm_precompute")
+        fused += _fused_launcher("rhs_eval_rk4_substep_g%d" % k, "const rfm_struct *rfmstruct, "
+                                 "const paramstruct *params, const double *auxevol_gfs, const double *in_gfs",
+                                 "*params, *rfmstruct, auxevol_gfs, in_gfs", "InteriorPoints")
+    print("(BENCH) rhs_eval codegen %.1fs (%d groups)" % (time.time() - t0, len(groups)), flush=True)
+    calls = "".join("  if ((rc = rhs_eval_g%d(rfmstruct, params, auxevol_gfs, in_gfs, rhs_gfs, stream))) return rc;\n"
+                    % k for k in range(len(groups)))
+    rhs_src += ("/* Evaluate the BSSN RHSs: the output gridfunctions are computed by %d kernels, each with\n"
+                " * the derivatives and intermediates its outputs need, to bound register pressure. */\n"
+                'extern "C" int rhs_eval(const rfm_struct *rfmstruct, const paramstruct *params, '
+                "const double *auxevol_gfs, const double *in_gfs, double *rhs_gfs, hipStream_t stream) {\n"
+                "  int rc;\n%s  return NRPY_OK;\n}\n" % (len(groups), calls))
+    calls = "".join("  if ((rc = rhs_eval_rk4_substep_g%d(S, rfmstruct, params, auxevol_gfs, in_gfs, y_n, rt, out, "
+                    "dt, stream))) return rc;\n" % k for k in range(len(groups)))
+    fused += ('extern "C" int rhs_eval_rk4_substep(const int S, const rfm_struct *rfmstruct, const paramstruct *params, '
+              "const double *auxevol_gfs, const double *in_gfs, double *y_n, double *rt, double *out, "
+              "const double dt, hipStream_t stream) {\n  int rc;\n%s  return NRPY_OK;\n}\n" % calls)
+    rhs_body = "\n".join("")
     t0 = time.time()
     ricci_body = fin.FD_outputC("returnstring",
                                 [lhrh(lhs=gri.gfaccess("auxevol_gfs", "RbarDD%d%d" % (i, j)), rhs=RbarDD[i][j])
@@ -256,7 +298,10 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     parts = [pre, bd.coord_host_functions(CoordSystem), '#include "../nrpy_runtime.h"', rfm_code,
              ricci_src, rhs_src, fused, h_src, detg_src, id_src,
              _mol_glue(True, True, None, "rfmstruct, params, auxevol_gfs, IN")]
-    meta["cost"] = {"rhs_eval": bd.count_body(rhs_body), "Ricci_eval": bd.count_body(ricci_body),
+    meta["rhs_groups"] = groups
+    meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
+    meta["cost"] = {"rhs_eval": {"flops": sum(c["flops"] for c in group_costs), "groups": group_costs},
+                    "Ricci_eval": bd.count_body(ricci_body),
                     "Hamiltonian_constraint": bd.count_body(h_body),
                     "enforce_detgammabar_constraint": bd.count_body(detg_body)}
     meta.update({"bundle": bundle_name, "CoordSystem": CoordSystem, "symmetry_axes": symmetry_axes,

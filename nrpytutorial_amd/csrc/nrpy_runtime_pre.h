@@ -41,3 +41,54 @@
 #define IDX4ptS(g, idx) ((size_t)(g) * ((size_t)Nxx_plus_2NGHOSTS0 * Nxx_plus_2NGHOSTS1 * Nxx_plus_2NGHOSTS2) + (size_t)(idx))
 
 #define NRPY_HIP_OK(call) ((call) == hipSuccess)
+
+// Launch geometry. Kernels run on a 1-D grid of blocks of shape
+// (BX, BY, BZ) tiling the (n0, n1, n2) index box. The hardware hands block b
+// to XCD b % 8, so consecutive blocks land on different L2 caches; the remap
+// below gives each XCD a contiguous range of tiles (i2 slowest), so stencil
+// neighbours in i1 and i2 are mostly resident in the same XCD's L2.
+#define NRPY_NUM_XCD 8
+#define NRPY_BLOCK_COORDS(n0, n1, n2)                                                          \
+  int nrpy_bx, nrpy_by, nrpy_bz;                                                               \
+  {                                                                                            \
+    const int nbx_ = ((n0) + (int)blockDim.x - 1) / (int)blockDim.x;                           \
+    const int nby_ = ((n1) + (int)blockDim.y - 1) / (int)blockDim.y;                           \
+    const int nb_ = (int)gridDim.x;                                                            \
+    const int b_ = (int)blockIdx.x;                                                            \
+    const int q_ = nb_ / NRPY_NUM_XCD, r_ = nb_ % NRPY_NUM_XCD;                                \
+    const int x_ = b_ % NRPY_NUM_XCD, l_ = b_ / NRPY_NUM_XCD;                                  \
+    const int t_ = (x_ < r_) ? x_ * (q_ + 1) + l_ : r_ * (q_ + 1) + (x_ - r_) * q_ + l_;      \
+    nrpy_bx = t_ % nbx_;                                                                       \
+    nrpy_by = (t_ / nbx_) % nby_;                                                              \
+    nrpy_bz = t_ / (nbx_ * nby_);                                                              \
+  }
+
+// Per-kernel block shapes, settable at run time (nrpy_set_block_shape) so an
+// autotuner can pick them; index by NRPY_KERNEL_* ids defined per bundle.
+#define NRPY_MAX_KERNELS 64
+static int nrpy_block_shape[NRPY_MAX_KERNELS][3];
+static inline dim3 nrpy_block(int k, int dx, int dy, int dz) {
+  if (k >= 0 && k < NRPY_MAX_KERNELS && nrpy_block_shape[k][0] > 0)
+    return dim3(nrpy_block_shape[k][0], nrpy_block_shape[k][1], nrpy_block_shape[k][2]);
+  return dim3(dx, dy, dz);
+}
+static inline unsigned nrpy_nblocks(dim3 b, int n0, int n1, int n2) {
+  return (unsigned)(((n0 + (int)b.x - 1) / (int)b.x) * ((n1 + (int)b.y - 1) / (int)b.y) *
+                    ((n2 + (int)b.z - 1) / (int)b.z));
+}
+// Set the block shape of kernel id k (threads per block must be <= 256 and a
+// multiple of 64); (0,0,0) restores the default. Returns 0 or NRPY_ERR_PARAM.
+extern "C" int nrpy_set_block_shape(int k, int bx, int by, int bz) {
+  if (k < 0 || k >= NRPY_MAX_KERNELS) return NRPY_ERR_PARAM;
+  if (bx == 0 && by == 0 && bz == 0) {
+    nrpy_block_shape[k][0] = nrpy_block_shape[k][1] = nrpy_block_shape[k][2] = 0;
+    return NRPY_OK;
+  }
+  if (bx < 1 || by < 1 || bz < 1) return NRPY_ERR_PARAM;
+  const int t = bx * by * bz;
+  if (t > 256 || t % 64 != 0) return NRPY_ERR_PARAM;
+  nrpy_block_shape[k][0] = bx;
+  nrpy_block_shape[k][1] = by;
+  nrpy_block_shape[k][2] = bz;
+  return NRPY_OK;
+}

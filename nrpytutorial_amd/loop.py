@@ -9,14 +9,30 @@ reference-metric tables, and ``EnableSIMD`` / ``DisableOpenMP`` /
 
 Instead of a triple loop, the body runs once per lane: i0 (the fastest memory
 direction) maps to threadIdx.x so a 64-lane wavefront reads 64 consecutive
-doubles of every gridfunction, i1 to threadIdx.y and i2 to blockIdx.z.
+doubles of every gridfunction; blocks tile (i0, i1, i2) with a shape set at
+launch time (nrpy_set_block_shape) and are remapped so that each XCD works on
+a contiguous part of the grid.
 ``loop1D`` / ``loop`` produce plain C loops for host-side code.
 """
 import re
 from . import NRPy_param_funcs as par
 
+# Default tile: one wavefront along i0, four i2 planes (measured best for the
+# BSSN kernels at 192^3 with the XCD remap: neighbours in i2 come from the same
+# block's other waves).
 BLOCK_X = 64
-BLOCK_Y = 4
+BLOCK_Y = 1
+BLOCK_Z = 4
+# every kernel is compiled for blocks of up to this many threads (the tunable shapes obey it)
+MAX_THREADS = 256
+# kernel name -> id into the runtime's tunable block-shape table
+KERNEL_IDS = {}
+
+
+def kernel_id(name):
+    if name not in KERNEL_IDS:
+        KERNEL_IDS[name] = len(KERNEL_IDS)
+    return KERNEL_IDS[name]
 
 
 def loop1D(idx_var="i", lower_bound="0", upper_bound="N", increment="1", pragma="#pragma omp parallel for"):
@@ -82,10 +98,13 @@ def simple_loop(options, interior):
     """Kernel body: lane -> gridpoint mapping, bounds check, coordinate/table reads, ``interior``."""
     o = parse_loopopts(options)
     lo = "NGHOSTS" if o["range"] == "interior" else "0"
-    s = ""
-    for d, src in ((0, "threadIdx.x + blockIdx.x*blockDim.x"), (1, "threadIdx.y + blockIdx.y*blockDim.y"),
-                   (2, "blockIdx.z")):
-        s += "const int i%d = %s + (%s);\n" % (d, lo, src)
+    n = ("Nxx%d" if o["range"] == "interior" else "Nxx_plus_2NGHOSTS%d")
+    # One-dimensional grid of blocks; NRPY_BLOCK_COORDS remaps the hardware's
+    # round-robin block-to-XCD assignment to contiguous tiles per XCD and
+    # decodes the (bx, by, bz) tile of this block.
+    s = "NRPY_BLOCK_COORDS(%s, %s, %s);\n" % (n % 0, n % 1, n % 2)
+    for d, t in ((0, "x"), (1, "y"), (2, "z")):
+        s += "const int i%d = %s + nrpy_b%s*(int)blockDim.%s + (int)threadIdx.%s;\n" % (d, lo, t, t, t)
     if o["range"] == "interior":
         s += "if (i0 >= Nxx0 + NGHOSTS || i1 >= Nxx1 + NGHOSTS || i2 >= Nxx2 + NGHOSTS) return;\n"
     else:
@@ -153,7 +172,7 @@ def kernel_and_launcher(name, desc, args, preloop, body, loopopts, postloop):
             unpack += "  const int %s = params.%s;\n" % (n, n)
     kname = name + "__gpu"
     kernel = ("/* %s */\n__global__ void __launch_bounds__(%d) %s(%s) {\n%s%s}\n" %
-              (desc, BLOCK_X * BLOCK_Y, kname, ", ".join(kargs), unpack,
+              (desc, MAX_THREADS, kname, ", ".join(kargs), unpack,
                "".join("  " + l + "\n" for l in full.splitlines())))
     rng = "interior" if o["range"] == "interior" else "all"
     if rng == "interior":
@@ -161,16 +180,17 @@ def kernel_and_launcher(name, desc, args, preloop, body, loopopts, postloop):
     else:
         n0, n1, n2 = "params->Nxx_plus_2NGHOSTS0", "params->Nxx_plus_2NGHOSTS1", "params->Nxx_plus_2NGHOSTS2"
     ptr_checks = " || ".join("!" + c for c in call if not c.startswith("*") and "xx[" not in c) or "0"
+    kid = kernel_id(name)
     launcher = (
         'extern "C" int %s(%s, hipStream_t stream) {\n'
         "  if (!params) return NRPY_ERR_NULL;\n"
         "  if (%s) return NRPY_ERR_NULL;\n"
         "  if (nrpy_check_params(params)) return NRPY_ERR_SHAPE;\n"
-        "  const dim3 block(%d, %d, 1);\n"
-        "  const dim3 grid((%s + %d - 1) / %d, (%s + %d - 1) / %d, %s);\n"
+        "  const dim3 block = nrpy_block(%d, %d, %d, %d);\n"
+        "  const dim3 grid(nrpy_nblocks(block, %s, %s, %s));\n"
         "  hipLaunchKernelGGL(%s, grid, block, 0, stream, %s);\n"
         "  return hipGetLastError() == hipSuccess ? 0 : NRPY_ERR_LAUNCH;\n"
-        "}\n" % (name, ", ".join(largs), ptr_checks, BLOCK_X, BLOCK_Y, n0, BLOCK_X, BLOCK_X, n1, BLOCK_Y, BLOCK_Y,
-                 n2, kname, ", ".join(call)))
+        "}\n" % (name, ", ".join(largs), ptr_checks, kid, BLOCK_X, BLOCK_Y, BLOCK_Z, n0, n1, n2, kname,
+                 ", ".join(call)))
     proto = 'extern "C" int %s(%s, hipStream_t stream);' % (name, ", ".join(largs))
     return kernel, launcher, proto
