@@ -1,0 +1,56 @@
+/* BSSN in the rescaled reference-metric formulation (bundles bssn_cartesian,
+ * bssn_spherical_sym2). Entry points of
+ * Tutorial-Start_to_Finish-BSSNCurvilinear-Two_BHs_Collide.ipynb's C driver;
+ * common declarations in nrpy_amd.h. Evolved gridfunctions in alphabetical
+ * order (aDD00..22, alpha, betU0..2, cf, hDD00..22, lambdaU0..2, trK,
+ * vetU0..2), auxevol RbarDD00..22, aux H. */
+#ifndef NRPY_AMD_BSSN_H
+#define NRPY_AMD_BSSN_H
+#include "nrpy_amd.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Replaces Ricci_eval(rfm_struct*, const paramstruct*, const REAL *in_gfs, REAL *auxevol_gfs)
+ * (Two_BHs_Collide.ipynb:486-540; BSSN_quantities.py:260-425): the six
+ * RbarDD components into auxevol_gfs at interior points. */
+int Ricci_eval(const rfm_struct *rfmstruct, const paramstruct *params, const double *in_gfs, double *auxevol_gfs,
+               hipStream_t stream);
+
+/* Replaces rhs_eval(rfm_struct*, const paramstruct*, const REAL *auxevol_gfs,
+ * const REAL *in_gfs, REAL *rhs_gfs) (Two_BHs_Collide.ipynb:436-484): BSSN +
+ * gauge RHSs (1+log lapse, covariant Gamma-driving shift) with FD4, upwinding
+ * on beta^i, RbarDD read from auxevol_gfs. Runs as rhs_eval_g0, rhs_eval_g1,
+ * ... (one kernel per output group, listed in <bundle>.json "rhs_groups"). */
+int rhs_eval(const rfm_struct *rfmstruct, const paramstruct *params, const double *auxevol_gfs,
+             const double *in_gfs, double *rhs_gfs, hipStream_t stream);
+int rhs_eval_g0(const rfm_struct *rfmstruct, const paramstruct *params, const double *auxevol_gfs,
+                const double *in_gfs, double *rhs_gfs, hipStream_t stream);
+int rhs_eval_g1(const rfm_struct *rfmstruct, const paramstruct *params, const double *auxevol_gfs,
+                const double *in_gfs, double *rhs_gfs, hipStream_t stream);
+int rhs_eval_rk4_substep_g0(const int S, const rfm_struct *rfmstruct, const paramstruct *params,
+                            const double *auxevol_gfs, const double *in_gfs, double *y_n, double *rt, double *out,
+                            const double dt, hipStream_t stream);
+int rhs_eval_rk4_substep_g1(const int S, const rfm_struct *rfmstruct, const paramstruct *params,
+                            const double *auxevol_gfs, const double *in_gfs, double *y_n, double *rt, double *out,
+                            const double dt, hipStream_t stream);
+
+/* Replaces Hamiltonian_constraint(rfm_struct*, const paramstruct*, REAL *in_gfs, REAL *aux_gfs)
+ * (Two_BHs_Collide.ipynb:542-556; BSSN_constraints.py:49-80). */
+int Hamiltonian_constraint(const rfm_struct *rfmstruct, const paramstruct *params, const double *in_gfs,
+                           double *aux_gfs, hipStream_t stream);
+
+/* Replaces enforce_detgammabar_constraint(const rfm_struct*, const paramstruct*, REAL *in_gfs)
+ * (Enforce_Detgammabar_Constraint.py:60-75), at all points, in place. */
+int enforce_detgammabar_constraint(const rfm_struct *rfmstruct, const paramstruct *params, double *in_gfs,
+                                   hipStream_t stream);
+
+/* Replaces initial_data(const paramstruct*, REAL *xx[3], REAL *in_gfs)
+ * (BSSN_ID_function_string.py:9-39) for Brill-Lindquist data
+ * (BrillLindquist.py:36-82); xx are device coordinate arrays. */
+int initial_data(const paramstruct *params, double *xx[3], double *in_gfs, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

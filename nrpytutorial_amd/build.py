@@ -114,7 +114,8 @@ def build_all(names=None, oracle=True, force=False):
         build_oracle([n for n in names if n in ORACLE_BUNDLES], force=force)
 
 
-ORACLE_BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2")
+ORACLE_BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2", "scalarwave_sinhsphericalv2",
+                  "scalarwave_sinhcylindricalv2")
 
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("-")]

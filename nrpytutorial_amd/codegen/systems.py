@@ -10,6 +10,11 @@ Start-to-Finish drivers do.
   enforce_detgammabar_constraint (Enforce_Detgammabar_Constraint.py:60-75),
   Brill-Lindquist initial_data (BSSN_ID_function_string.py:9-39), rfm
   precompute on, symmetry axis "2" for the axisymmetric Spherical runs.
+* ``scalarwave_sinhsphericalv2`` / ``scalarwave_sinhcylindricalv2``:
+  Tutorial-Start_to_Finish-ScalarWaveCurvilinear.ipynb (ScalarWaveCurvilinear_RHSs
+  with rfm precompute, plane-wave data) in the two coordinate systems of the
+  reference's curvilinear boundary-condition validation
+  (Tutorial-Start_to_Finish-Curvilinear_BCs.ipynb).
 * ``maxwell_cartesian``: VacuumMaxwell_Flat_Evol_Cartesian System II.
 
 Each bundle is generated in a fresh interpreter (the registries are global):
@@ -27,7 +32,9 @@ from .. import outputC as oc
 from ..outputC import lhrh
 from . import bundle as bd
 
-BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2")
+CURVI_SCALARWAVE = {"scalarwave_sinhsphericalv2": "SinhSphericalv2",
+                    "scalarwave_sinhcylindricalv2": "SinhCylindricalv2"}
+BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2") + tuple(CURVI_SCALARWAVE)
 
 
 def _mol_glue(has_ricci, has_detg, rhs_args_decl, rhs_call_args):
@@ -186,6 +193,55 @@ def scalarwave_cartesian():
     return bd.write_bundle("scalarwave_cartesian", parts, meta)
 
 
+def scalarwave_curvilinear(CoordSystem, bundle_name, NGHOSTS=3):
+    """Scalar wave in a curvilinear reference metric with precomputed rfm tables
+    (Tutorial-Start_to_Finish-ScalarWaveCurvilinear.ipynb: ScalarWaveCurvilinear_RHSs, plane-wave
+    data, quadratic extrapolation + parity boundary conditions). NGHOSTS=3 is the ghost depth of
+    the reference's curvilinear boundary-condition validation (FD4 + 1)."""
+    from ..ScalarWave import ScalarWaveCurvilinear_RHSs as sw, InitialData as swid
+    par.set_parval_from_str("grid::DIM", 3)
+    par.set_parval_from_str("finite_difference::FD_CENTDERIVS_ORDER", 4)
+    par.set_parval_from_str("reference_metric::CoordSystem", CoordSystem)
+    rfm.reference_metric()
+    swid.InitialData(Type="PlaneWave", CoordSystem=CoordSystem)
+    id_rhss = [swid.uu_ID, swid.vv_ID]
+    par.set_parval_from_str("reference_metric::enable_rfm_precompute", "True")
+    rfm.reference_metric()
+    sw.ScalarWaveCurvilinear_RHSs()
+    exprs = [lhrh(lhs=gri.gfaccess("rhs_gfs", "uu"), rhs=sw.uu_rhs),
+             lhrh(lhs=gri.gfaccess("rhs_gfs", "vv"), rhs=sw.vv_rhs)]
+    body = fin.FD_outputC("returnstring", exprs, params="outCverbose=False")
+    rhs_src = oc.outCfunction(outfile="returnstring", desc="Evaluate the scalar wave RHSs, curvilinear coordinates",
+                              name="rhs_eval",
+                              params="rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
+                                     "const REAL *restrict in_gfs,REAL *restrict rhs_gfs",
+                              body=body, loopopts="InteriorPoints,Enable_rfm_precompute")
+    args_decl = ("const paramstruct params, const rfm_struct rfmstruct, const double *__restrict__ auxevol_gfs, "
+                 "const double *__restrict__ in_gfs")
+    fused = bd.fused_rk4_kernel("rhs_eval_rk4_substep", args_decl, "", body,
+                                loopopts="InteriorPoints,Enable_rfm_precompute")
+    fused += _fused_launcher("rhs_eval_rk4_substep", "const rfm_struct *rfmstruct, const paramstruct *params, "
+                             "const double *auxevol_gfs, const double *in_gfs",
+                             "*params, *rfmstruct, auxevol_gfs, in_gfs", "InteriorPoints")
+    rfm_code = bd.rfm_struct_code()
+    tables = [(n, str(e), list(a)) for n, e, a in rfm.rfm_precompute_tables()]
+    par.set_parval_from_str("reference_metric::enable_rfm_precompute", "False")
+    rfm.ref_metric__hatted_quantities()
+    id_src = _initial_data_kernel(id_rhss, ["uu", "vv"])
+    # The EigenCoord's reference metric registers its own C parameters (e.g. RMAX), so
+    # the coordinate functions are generated before the paramstruct is declared.
+    coords = bd.coord_host_functions(CoordSystem)
+    pre, meta = _header(NGHOSTS)
+    parts = [pre, coords, '#include "../nrpy_runtime.h"', rfm_code,
+             rhs_src, fused, id_src, _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN")]
+    meta["cost"] = {"rhs_eval": bd.count_body(body)}
+    meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
+    meta.update({"bundle": bundle_name, "family": "scalarwave_curvilinear", "CoordSystem": CoordSystem,
+                 "rfm_tables": tables,
+                 "kernels": ["rhs_eval", "initial_data"], "has_ricci": False, "has_detg": False})
+    return bd.write_bundle(bundle_name, parts, meta)
+
+
 def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     from ..BSSN import BSSN_quantities as Bq, BSSN_RHSs as rhs, BSSN_gauge_RHSs as gaugerhs
     from ..BSSN import Enforce_Detgammabar_Constraint as EGC, BSSN_constraints as bssncon
@@ -261,7 +317,6 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     fused += ('extern "C" int rhs_eval_rk4_substep(const int S, const rfm_struct *rfmstruct, const paramstruct *params, '
               "const double *auxevol_gfs, const double *in_gfs, double *y_n, double *rt, double *out, "
               "const double dt, hipStream_t stream) {\n  int rc;\n%s  return NRPY_OK;\n}\n" % calls)
-    rhs_body = "\n".join("")
     t0 = time.time()
     ricci_body = fin.FD_outputC("returnstring",
                                 [lhrh(lhs=gri.gfaccess("auxevol_gfs", "RbarDD%d%d" % (i, j)), rhs=RbarDD[i][j])
@@ -318,6 +373,8 @@ def generate(name):
         return bssn("Cartesian", "", name)
     if name == "bssn_spherical_sym2":
         return bssn("Spherical", "2", name)
+    if name in CURVI_SCALARWAVE:
+        return scalarwave_curvilinear(CURVI_SCALARWAVE[name], name)
     if name == "maxwell_cartesian":
         from . import maxwell
         return maxwell.maxwell_cartesian()

@@ -176,7 +176,14 @@ def symm_matrix_inverter3x3(a):
     c11 = a[0][0] * a[2][2] - a[0][2] ** 2
     c12 = a[0][1] * a[0][2] - a[0][0] * a[1][2]
     c22 = a[0][0] * a[1][1] - a[0][1] ** 2
-    det = a[0][0] * c00 + a[0][1] * c01 + a[0][2] * c02
+    # Determinant as the fully expanded Leibniz sum (six terms, two pairs equal by
+    # symmetry). Kept expanded rather than as the cofactor row expansion: for
+    # symbolic entries with fractional powers, (det)^(-1/6)-type expressions take
+    # their branch from the exact expression tree, and the expanded polynomial is
+    # the form the reference's trusted values were computed from
+    # (indexedexp.py:331-336).
+    det = (a[0][0] * a[1][1] * a[2][2] + 2 * a[0][1] * a[0][2] * a[1][2]
+           - a[0][0] * a[1][2] ** 2 - a[1][1] * a[0][2] ** 2 - a[2][2] * a[0][1] ** 2)
     if det == 0:
         raise NonInvertibleMatrixError("matrix has determinant zero")
     inv = [[c00 / det, c01 / det, c02 / det],

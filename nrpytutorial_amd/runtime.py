@@ -76,6 +76,7 @@ PROTOTYPES = {
 KERNEL_PROTOTYPES = {
     # reference-style kernels: (params..., stream)
     "scalarwave": {"rhs_eval": [_VP, _VP, _VP, _VP]},
+    "scalarwave_curvilinear": {"rhs_eval": [_VP, _VP, _VP, _VP, _VP]},
     "bssn": {"rhs_eval": [_VP, _VP, _VP, _VP, _VP, _VP], "Ricci_eval": [_VP, _VP, _VP, _VP, _VP],
              "Hamiltonian_constraint": [_VP, _VP, _VP, _VP, _VP],
              "enforce_detgammabar_constraint": [_VP, _VP, _VP, _VP]},
@@ -114,7 +115,7 @@ class Bundle:
             f = getattr(self.lib, sym)
             f.restype = res
             f.argtypes = args
-        family = name.split("_")[0]
+        family = self.meta.get("family", name.split("_")[0])
         for sym, args in KERNEL_PROTOTYPES.get(family, {}).items():
             f = getattr(self.lib, sym)
             f.restype = _I

@@ -349,6 +349,35 @@ def scalarwave_cartesian():
                                           "evol": evol, "evol_parity": parity_list(evol)})
 
 
+def scalarwave_curvilinear(CoordSystem, name):
+    """Scalar wave in a curvilinear reference metric, closed-form rfm at each point
+    (ScalarWave/ScalarWaveCurvilinear_RHSs.py:43-98, plane-wave data ScalarWave/InitialData.py:59-88)."""
+    from nrpytutorial_amd.ScalarWave import ScalarWaveCurvilinear_RHSs as sw, InitialData as swid
+    from nrpytutorial_amd.outputC import lhrh
+    par.set_parval_from_str("reference_metric::CoordSystem", CoordSystem)
+    rfm.reference_metric()
+    swid.InitialData(Type="PlaneWave", CoordSystem=CoordSystem)
+    sw.ScalarWaveCurvilinear_RHSs()
+    rhs = point_body([lhrh("rhs_gfs[IDX4S(UUGF,i0,i1,i2)]", sw.uu_rhs), lhrh("rhs_gfs[IDX4S(VVGF,i0,i1,i2)]", sw.vv_rhs)])
+    rep, red = sp.cse([swid.uu_ID, swid.vv_ID], sp.numbered_symbols("tmp_"))
+    idl = ["const double %s = %s;" % (a, cexpr(b)) for a, b in rep]
+    idl += ["in_gfs[IDX4S(UUGF,i0,i1,i2)] = %s;" % cexpr(red[0]), "in_gfs[IDX4S(VVGF,i0,i1,i2)] = %s;" % cexpr(red[1])]
+    coords = coord_functions(CoordSystem)  # registers the EigenCoord's C parameters (e.g. RMAX)
+    parts = [header(3), paramstruct_c(), coords, '#include "../oracle_runtime.h"',
+             cfunc("oracle_rhs_eval", "const paramstruct *params, double *xx[3], const double *auxevol_gfs, "
+                   "const double *in_gfs, double *rhs_gfs", rhs, "interior",
+                   "curvilinear scalar wave RHS (ScalarWave/ScalarWaveCurvilinear_RHSs.py:43-98)"),
+             "void oracle_Ricci_eval(const paramstruct *params, double *xx[3], const double *in_gfs, double *auxevol_gfs)"
+             " { (void)params; (void)xx; (void)in_gfs; (void)auxevol_gfs; }",
+             "void oracle_enforce_detgammabar_constraint(const paramstruct *params, double *xx[3], double *in_gfs)"
+             " { (void)params; (void)xx; (void)in_gfs; }",
+             cfunc("oracle_initial_data", "const paramstruct *params, double *xx[3], double *in_gfs", idl, "all",
+                   "plane wave in curvilinear coordinates (ScalarWave/InitialData.py:59-88)"),
+             '#include "../oracle_mol.h"']
+    evol = gri.gridfunction_lists()[0]
+    write(name, parts, {"paramstruct": _members_with_defaults(), "evol": evol, "evol_parity": parity_list(evol)})
+
+
 def bssn(CoordSystem, symmetry_axes, name):
     from nrpytutorial_amd.BSSN import BSSN_quantities as Bq, BSSN_RHSs as rhs, BSSN_gauge_RHSs as gaugerhs
     from nrpytutorial_amd.BSSN import Enforce_Detgammabar_Constraint as EGC, BSSN_constraints as bssncon
@@ -411,6 +440,10 @@ def bssn(CoordSystem, symmetry_axes, name):
                         "evol_parity": parity_list(evol)})
 
 
+CURVI_SCALARWAVE = {"scalarwave_sinhsphericalv2": "SinhSphericalv2",
+                    "scalarwave_sinhcylindricalv2": "SinhCylindricalv2"}
+
+
 if __name__ == "__main__":
     for b in sys.argv[1:]:
         if b == "scalarwave_cartesian":
@@ -419,6 +452,8 @@ if __name__ == "__main__":
             bssn("Cartesian", "", b)
         elif b == "bssn_spherical_sym2":
             bssn("Spherical", "2", b)
+        elif b in CURVI_SCALARWAVE:
+            scalarwave_curvilinear(CURVI_SCALARWAVE[b], b)
         else:
             raise SystemExit("unknown oracle bundle " + b)
         print("oracle generated", b, flush=True)

@@ -1,0 +1,107 @@
+"""The C ABI: every bundle library loads (no GPU needed to load it) and exports
+every entry point that include/*.h declares for its system; the headers
+compile as C and as C++; status codes and the block-shape setter behave."""
+import ctypes
+import glob
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+from nrpytutorial_amd import runtime as rt
+from nrpytutorial_amd.codegen.systems import BUNDLES
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+
+FAMILY_HEADER = {"scalarwave": "nrpy_amd_scalarwave.h",
+                 "scalarwave_curvilinear": "nrpy_amd_scalarwave_curvilinear.h",
+                 "bssn": "nrpy_amd_bssn.h"}
+
+
+def declared(header):
+    text = open(os.path.join(INC, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^(?:int|void|double)\s+\**([A-Za-z_][A-Za-z0-9_]*)\s*\(", text, flags=re.M)))
+
+
+def family(bundle):
+    meta = rt.load_meta(bundle)
+    return meta.get("family", bundle.split("_")[0])
+
+
+def test_every_header_is_used():
+    used = {"nrpy_amd.h"} | set(FAMILY_HEADER.values())
+    assert {os.path.basename(p) for p in glob.glob(os.path.join(INC, "*.h"))} == used
+
+
+@pytest.mark.parametrize("bundle", BUNDLES)
+def test_library_exports_declared_symbols(bundle):
+    path = os.path.join(rt.LIBDIR, "libnrpy_%s.so" % bundle)
+    lib = ctypes.CDLL(path)
+    want = declared("nrpy_amd.h") + declared(FAMILY_HEADER[family(bundle)])
+    missing = [s for s in want if not hasattr(lib, s)]
+    assert not missing, missing
+    assert len(want) > 20
+
+
+@pytest.mark.parametrize("lang,compiler", [("c", "gcc"), ("c++", "g++")])
+def test_headers_compile(lang, compiler, tmp_path):
+    src = tmp_path / ("t." + ("c" if lang == "c" else "cpp"))
+    for h in FAMILY_HEADER.values():
+        src.write_text('#include "%s"\nint main(void) { return NRPY_OK; }\n' % h)
+        r = subprocess.run([compiler, "-fsyntax-only", "-D__HIP_PLATFORM_AMD__", "-I", INC, "-I", "/opt/rocm/include",
+                            str(src)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-2000:]
+
+
+def test_status_codes_match_runtime():
+    text = open(os.path.join(INC, "nrpy_amd.h")).read()
+    codes = {int(v): k for k, v in re.findall(r"#define (NRPY_ERR_\w+) (-\d+)", text)}
+    assert sorted(codes) == sorted(rt.NRPY_ERRORS)
+    pre = open(os.path.join(ROOT, "nrpytutorial_amd", "csrc", "nrpy_runtime_pre.h")).read()
+    for v, k in codes.items():
+        assert re.search(r"#define %s %d\b" % (k, v), pre), k
+
+
+def test_block_shape_setter_validates():
+    b = rt.Bundle("bssn_cartesian")
+    f = b.lib.nrpy_set_block_shape
+    assert f(0, 64, 1, 4) == 0
+    assert f(0, 0, 0, 0) == 0
+    assert f(0, 64, 2, 4) == -4       # 512 threads
+    assert f(0, 48, 1, 1) == -4       # not a multiple of 64
+    assert f(99, 64, 1, 1) == -4
+    assert f(-1, 64, 1, 1) == -4
+
+
+def test_null_and_shape_errors_on_host():
+    """Error paths that return before touching the device."""
+    b = rt.Bundle("scalarwave_cartesian")
+    L = b.lib
+    n = (ctypes.c_int * 3)(8, 8, 8)
+    xx = (ctypes.c_void_p * 3)()
+    assert L.set_Nxx_dxx_invdx_params__and__xx(0, n, None, xx) == -1
+    p = b.new_params()
+    assert L.set_Nxx_dxx_invdx_params__and__xx(2, n, ctypes.byref(p), xx) == -4
+    bad = (ctypes.c_int * 3)(8, 0, 8)
+    assert L.set_Nxx_dxx_invdx_params__and__xx(0, bad, ctypes.byref(p), xx) == -2
+    h = ctypes.c_void_p()
+    assert L.bcstruct_create_host(None, xx, ctypes.byref(h)) == -1
+    assert L.MoL_graph_launch(None, 1, None) == -1
+
+
+def test_paramstruct_layout_matches_meta():
+    """The ctypes mirror built from <bundle>.json has the size of the C struct
+    (members sorted by name, NRPy_param_funcs.py:241-242)."""
+    for bundle in BUNDLES:
+        meta = rt.load_meta(bundle)
+        names = [m[0] for m in meta["paramstruct"]]
+        assert names == sorted(names)
+        src = open(os.path.join(rt.GENDIR, bundle + ".hip")).read()
+        body = src[src.index("typedef struct __paramstruct__ {"):]
+        body = body[:body.index("} paramstruct;")]
+        cnames = re.findall(r"^\s+(?:int|double) (\w+);", body, flags=re.M)
+        assert cnames == names, bundle

@@ -1,0 +1,89 @@
+"""The C oracle's method-of-lines evolution against closed forms (CPU).
+
+* RK4 amplification: one and three RK4 steps of a plane wave equal the closed
+  form of tests/mol_closed_form.py at every point the boundary conditions
+  cannot reach.
+* Convergence to the exact plane wave u = 2 + sin(k.x - ct) at fourth order
+  (the reference's Start-to-Finish ScalarWave validation,
+  Tutorial-Start_to_Finish-ScalarWave.ipynb and
+  Tutorial-Start_to_Finish-ScalarWaveCurvilinear.ipynb: 16 -> 24 points and
+  the (16/24)^4 rescaling), in Cartesian and SinhSphericalv2 coordinates.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle, OracleGrid
+import mol_closed_form as cf
+
+KVEC = (0.37, -0.81, 0.46)
+
+
+def _plane_wave_grid(bundle, Nxx, **kw):
+    o = Oracle(bundle)
+    khat = cf.unit(KVEC)
+    p = dict(kk0=khat[0], kk1=khat[1], kk2=khat[2], wavespeed=1.0, **kw)
+    g = OracleGrid(o, Nxx, p)
+    y = g.zeros(2)
+    g.initial_data(y)
+    g.apply_bcs(y)
+    return o, g, y, khat
+
+
+@pytest.mark.parametrize("nsteps", [1, 3])
+def test_rk4_amplification_closed_form(nsteps):
+    Nxx = (16 * nsteps + 20, 16 * nsteps + 18, 16 * nsteps + 22)
+    o, g, y, khat = _plane_wave_grid("scalarwave_cartesian", Nxx)
+    dxx = (g.p.dxx0, g.p.dxx1, g.p.dxx2)
+    dt = 0.5 * min(dxx)
+    g.mol_step(y, dt, nsteps)
+    u, v = cf.plane_wave_after_rk4(g.xx, khat, dxx, 1.0, dt, nsteps)
+    sl = cf.pure_region(Nxx, o.NGHOSTS, nsteps)
+    assert np.max(np.abs(y[0][sl] - u[sl])) < 1e-13
+    assert np.max(np.abs(y[1][sl] - v[sl])) < 1e-13
+    # the closed form itself differs from the exact solution by the truncation error
+    t = nsteps * dt
+    X0, X1, X2 = g.xx[0][None, None, :], g.xx[1][None, :, None], g.xx[2][:, None, None]
+    exact = 2 + np.sin(khat[0] * X0 + khat[1] * X1 + khat[2] * X2 - t)
+    assert np.max(np.abs(y[0][sl] - exact[sl])) > 1e-9
+
+
+def _max_err(bundle, Nxx, T, CFL, region, **kw):
+    """Max |u - exact| at time T over ``region`` of the interior (a slice tuple in
+    (i2, i1, i0) order): the extrapolated outer boundary of an outgoing wave is a
+    low-order error source, so convergence is measured where it has not arrived."""
+    o, g, y, khat = _plane_wave_grid(bundle, Nxx, **kw)
+    dt = g.find_timestep(CFL)
+    n = int(round(T / dt))
+    dt = T / n
+    g.mol_step(y, dt, n)
+    g2 = OracleGrid(o, Nxx, dict(kk0=khat[0], kk1=khat[1], kk2=khat[2], wavespeed=1.0, time=T, **kw))
+    ex = g2.zeros(2)
+    g2.initial_data(ex)
+    return np.max(np.abs(g.interior(y[0])[region] - g.interior(ex[0])[region]))
+
+
+CART = dict(xmin=-3.0, xmax=3.0, ymin=-3.0, ymax=3.0, zmin=-3.0, zmax=3.0)
+
+
+def test_cartesian_fourth_order_convergence():
+    """Central quarter (per axis) of [-3,3]^3 after t=1; 16 points per wavelength at N=16."""
+    errs = []
+    for n in (16, 24, 32):
+        q = n // 8
+        errs.append(_max_err("scalarwave_cartesian", (n, n, n), 1.0, 0.5, (slice(n // 2 - q, n // 2 + q),) * 3,
+                             **CART))
+    r1 = np.log(errs[0] / errs[1]) / np.log(24 / 16)
+    r2 = np.log(errs[1] / errs[2]) / np.log(32 / 24)
+    assert 3.4 < r1 < 4.6 and 3.4 < r2 < 4.6, (errs, r1, r2)
+
+
+def test_sinhsphericalv2_fourth_order_convergence():
+    """Inner half in the radial index (all angles: the parity boundary conditions
+    at the axis and origin are in the region) after t=0.5, AMPL=3."""
+    errs = []
+    for n in (8, 12, 16):
+        errs.append(_max_err("scalarwave_sinhsphericalv2", (2 * n, n, 2 * n), 0.5, 1.0,
+                             (slice(None), slice(None), slice(0, n)), AMPL=3.0))
+    r1 = np.log(errs[0] / errs[1]) / np.log(12 / 8)
+    r2 = np.log(errs[1] / errs[2]) / np.log(16 / 12)
+    assert 3.4 < r1 < 4.6 and 3.4 < r2 < 4.6, (errs, r1, r2)
