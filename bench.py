@@ -19,6 +19,7 @@ reference's optimized flags, OpenMP over the host cores, on a bounded sample.
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 192] [--no-cpu]
 """
 import argparse
+import ctypes
 import json
 import os
 import subprocess
@@ -87,6 +88,21 @@ def cpu_baseline(n=64, min_seconds=12.0, max_steps=200):
                       % (n, res["steps"], res["seconds"], cores, model)}
 
 
+def measured_traffic(kernel, n, outputs):
+    """HBM bytes per launch of ``kernel`` at n^3 from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the plain-copy calibration), or None when
+    there is none for this kernel and output group."""
+    path = os.path.join(ROOT, "profiles", "%s_bssn%d_traffic.json" % (kernel, n))
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        if d.get("outputs") != list(outputs):
+            return None
+        return d["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def time_kernel(torch, fn, reps):
     """Average duration (s) of fn() launched on torch's current stream, HIP events."""
     fn()
@@ -129,6 +145,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-bl", action="store_true", help="skip the Brill-Lindquist 72x12x2 line")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device copy (scale for the HBM byte counters)")
     ap.add_argument("--cpu-baseline-worker", nargs=3)
     a = ap.parse_args()
     if a.cpu_baseline_worker:
@@ -173,28 +192,55 @@ def main():
     reps = 10
     cost = b.meta["cost"]
     npts = N ** 3
+    ngf = len(b.evol)
+    groups = b.meta["rhs_groups"]
     t_ricci = time_kernel(torch, lambda: check(b.lib.Ricci_eval(g.rfm, g.pp(), ptr(ev.y_n), ptr(ev.auxevol), s),
                                                "Ricci"), reps)
-    t_fused = time_kernel(torch, lambda: check(b.lib.rhs_eval_rk4_substep(
-        1, g.rfm, g.pp(), ptr(ev.auxevol), ptr(ev.k_odd), ptr(ev.y_n), ptr(ev.rt), ptr(ev.k_even), dt, s),
-        "rhs_eval_rk4_substep"), reps)
+    t_group = []
+    for k in range(len(groups)):
+        f = getattr(b.lib, "rhs_eval_rk4_substep_g%d" % k)
+        f.restype = ctypes.c_int
+        f.argtypes = b.lib.rhs_eval_rk4_substep.argtypes
+        t_group.append(time_kernel(torch, lambda f=f: check(f(
+            1, g.rfm, g.pp(), ptr(ev.auxevol), ptr(ev.k_odd), ptr(ev.y_n), ptr(ev.rt), ptr(ev.k_even), dt, s),
+            "rhs_eval_rk4_substep"), reps))
     rhs_tmp = ev.k_even
     t_rhs = time_kernel(torch, lambda: check(b.lib.rhs_eval(g.rfm, g.pp(), ptr(ev.auxevol), ptr(ev.y_n),
                                                             ptr(rhs_tmp), s), "rhs_eval"), reps)
     t_bc = time_kernel(torch, lambda: ev.apply_bcs(ev.k_odd, composed=True), reps)
     t_detg = time_kernel(torch, lambda: ev.post_bcs(ev.k_odd), reps)
-    ngf = len(b.evol)
+    # Dominant kernel: the slowest output group of the fused RHS + RK4 update (substep 1).
+    # Algorithmic traffic per interior point: every evolved gridfunction and the six RbarDD
+    # read once, and per output gridfunction y_n and the running total read, the stage
+    # output and the running total written.
+    kdom = max(range(len(groups)), key=lambda k: t_group[k])
+    n_out0 = len(groups[kdom])
+    dom_flops = cost["rhs_eval"]["groups"][kdom]["flops"] + 6 * n_out0
+    dom_bytes = 8 * (ngf + 6 + 4 * n_out0)
+    t_dom = t_group[kdom]
+    dom_name = "rhs_eval_rk4_substep_g%d" % kdom
+    dom_gbs = dom_bytes * npts / t_dom / 1e9
+    dom_tf = dom_flops * npts / t_dom / 1e12
     fused_flops = cost["rhs_eval"]["flops"] + 6 * ngf
-    fused_bytes = 8 * (ngf + 6 + ngf + ngf + ngf + ngf)  # read X, Rbar, y_n, rt; write out, rt (substep 1)
-    achieved_tf = fused_flops * npts / t_fused / 1e12
-    achieved_gbs = fused_bytes * npts / t_fused / 1e9
+    t_fused = sum(t_group)
     rhs_only_gp = npts / (t_ricci + t_rhs)
-    log("kernel avg: Ricci %.3f ms (%.1f TF/s), fused RHS+RK %.3f ms (%.1f TF/s, %.0f GB/s), rhs_eval %.3f ms, "
-        "ghost fill %.3f ms, detg %.3f ms" % (1e3 * t_ricci, cost["Ricci_eval"]["flops"] * npts / t_ricci / 1e12,
-                                              1e3 * t_fused, achieved_tf, achieved_gbs, 1e3 * t_rhs, 1e3 * t_bc,
-                                              1e3 * t_detg))
-    bl = bench_bl(torch)
-    log("BL 72x12x2: 50 steps in %.4f s -> %.3e gp/s, H-L2 %.6e" % (bl["seconds"], bl["gp_per_s"], bl["H_L2"]))
+    log("kernel avg: Ricci %.3f ms (%.2f TF/s), fused RHS+RK groups %s ms (slowest group: %.0f GB/s algorithmic, "
+        "%.2f TF/s), rhs_eval %.3f ms, ghost fill %.3f ms, detg %.3f ms"
+        % (1e3 * t_ricci, cost["Ricci_eval"]["flops"] * npts / t_ricci / 1e12,
+           "+".join("%.3f" % (1e3 * t) for t in t_group), dom_gbs, dom_tf, 1e3 * t_rhs, 1e3 * t_bc, 1e3 * t_detg))
+    traffic = measured_traffic(dom_name, N, groups[kdom])
+    if a.calibrate_copy:
+        # A plain device copy of known size, for scaling the HBM byte counters.
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d bytes read, %d bytes written per launch" % (src.numel() * 8, src.numel() * 8))
+    bl = None
+    if not a.no_bl:
+        bl = bench_bl(torch)
+        log("BL 72x12x2: 50 steps in %.4f s -> %.3e gp/s, H-L2 %.6e" % (bl["seconds"], bl["gp_per_s"], bl["H_L2"]))
 
     cpu = None
     if not a.no_cpu:
@@ -210,18 +256,24 @@ def main():
         "config": {"workload": "BSSN vacuum RK4 step, Cartesian reference metric, %d^3 interior, FD4 + upwinding, "
                                "1+log lapse, covariant Gamma-driver (eta=2), hipGraph-replayed substeps" % N,
                    "grid": [N, N, N], "NGHOSTS": 3, "gridfunctions": ngf, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "rhs_eval_rk4_substep (FP64 VALU; no matrix cores)",
-                     "flops_per_point": fused_flops, "bytes_per_point": fused_bytes,
-                     "avg_launch_ms": 1e3 * t_fused, "hbm_GBps_algorithmic": achieved_gbs},
+        "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "%s (fused BSSN RHS of %s + RK4 update, substep 1)"
+                               % (dom_name, ",".join(groups[kdom])),
+                     "bytes_per_point": dom_bytes, "points_per_launch": npts, "avg_launch_ms": 1e3 * t_dom,
+                     "flops_per_point": dom_flops, "achieved_fp64_TFLOPs": dom_tf,
+                     "fp64_frac": dom_tf / FP64_PEAK_TFLOPS,
+                     "note": "algorithmic intensity %.1f FLOP/B is below the FP64 ridge (%.1f FLOP/B), so HBM is "
+                             "the roof; FP64 VALU work, no matrix cores" % (dom_flops / dom_bytes,
+                                                                           FP64_PEAK_TFLOPS * 1e3 / HBM_PEAK_GBS)},
         "cpu_baseline": cpu,
-        "extra": {"rhs_only_gp_per_s": rhs_only_gp, "kernel_ms": {"Ricci_eval": 1e3 * t_ricci,
-                                                                  "rhs_eval_rk4_substep": 1e3 * t_fused,
-                                                                  "rhs_eval": 1e3 * t_rhs,
-                                                                  "ghost_fill": 1e3 * t_bc, "detg": 1e3 * t_detg},
-                  "ricci_flops_per_point": cost["Ricci_eval"]["flops"],
-                  "bl_72x12x2_50steps": bl},
+        "extra": {"rhs_only_gp_per_s": rhs_only_gp,
+                  "kernel_ms": {"Ricci_eval": 1e3 * t_ricci, "rhs_eval": 1e3 * t_rhs, "ghost_fill": 1e3 * t_bc,
+                                "detg": 1e3 * t_detg,
+                                **{"rhs_eval_rk4_substep_g%d" % k: 1e3 * t for k, t in enumerate(t_group)}},
+                  "fused_substep_fp64_TFLOPs": fused_flops * npts / t_fused / 1e12,
+                  "fused_flops_per_point": fused_flops, "ricci_flops_per_point": cost["Ricci_eval"]["flops"],
+                  "rhs_groups": groups, "bl_72x12x2_50steps": bl},
     }
     print(json.dumps(out))
     return 0

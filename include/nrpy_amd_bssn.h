@@ -16,12 +16,20 @@ extern "C" {
  * RbarDD components into auxevol_gfs at interior points. */
 int Ricci_eval(const rfm_struct *rfmstruct, const paramstruct *params, const double *in_gfs, double *auxevol_gfs,
                hipStream_t stream);
+/* Ricci_eval runs as Ricci_eval_g0 (RbarDD00, 01, 02) and Ricci_eval_g1 (RbarDD11, 12, 22)
+ * (groups listed in <bundle>.json "ricci_groups"). */
+int Ricci_eval_g0(const rfm_struct *rfmstruct, const paramstruct *params, const double *in_gfs, double *auxevol_gfs,
+                  hipStream_t stream);
+int Ricci_eval_g1(const rfm_struct *rfmstruct, const paramstruct *params, const double *in_gfs, double *auxevol_gfs,
+                  hipStream_t stream);
 
 /* Replaces rhs_eval(rfm_struct*, const paramstruct*, const REAL *auxevol_gfs,
  * const REAL *in_gfs, REAL *rhs_gfs) (Two_BHs_Collide.ipynb:436-484): BSSN +
  * gauge RHSs (1+log lapse, covariant Gamma-driving shift) with FD4, upwinding
- * on beta^i, RbarDD read from auxevol_gfs. Runs as rhs_eval_g0, rhs_eval_g1,
- * ... (one kernel per output group, listed in <bundle>.json "rhs_groups"). */
+ * on beta^i, RbarDD read from auxevol_gfs. Runs as rhs_eval_g0 (aDD, trK),
+ * rhs_eval_g1 (hDD, alpha, cf), rhs_eval_g2 (lambdaU, vetU, betU): one kernel per
+ * output group, listed in <bundle>.json "rhs_groups". The _rk4_substep_ variants
+ * fuse the RK4 update (see rhs_eval_rk4_substep in nrpy_amd.h). */
 int rhs_eval(const rfm_struct *rfmstruct, const paramstruct *params, const double *auxevol_gfs,
              const double *in_gfs, double *rhs_gfs, hipStream_t stream);
 int rhs_eval_g0(const rfm_struct *rfmstruct, const paramstruct *params, const double *auxevol_gfs,
@@ -32,6 +40,11 @@ int rhs_eval_rk4_substep_g0(const int S, const rfm_struct *rfmstruct, const para
                             const double *auxevol_gfs, const double *in_gfs, double *y_n, double *rt, double *out,
                             const double dt, hipStream_t stream);
 int rhs_eval_rk4_substep_g1(const int S, const rfm_struct *rfmstruct, const paramstruct *params,
+                            const double *auxevol_gfs, const double *in_gfs, double *y_n, double *rt, double *out,
+                            const double dt, hipStream_t stream);
+int rhs_eval_g2(const rfm_struct *rfmstruct, const paramstruct *params, const double *auxevol_gfs,
+                const double *in_gfs, double *rhs_gfs, hipStream_t stream);
+int rhs_eval_rk4_substep_g2(const int S, const rfm_struct *rfmstruct, const paramstruct *params,
                             const double *auxevol_gfs, const double *in_gfs, double *y_n, double *rt, double *out,
                             const double dt, hipStream_t stream);
 

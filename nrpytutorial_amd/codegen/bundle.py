@@ -271,9 +271,9 @@ def fused_rk4_kernel(name, args_decl, call_args, body_with_writes, loopopts="Int
     for n in ("Nxx0", "Nxx1", "Nxx2", "Nxx_plus_2NGHOSTS0", "Nxx_plus_2NGHOSTS1", "Nxx_plus_2NGHOSTS2"):
         if not re.search(r"const int %s = " % n, unpack):
             unpack += "  const int %s = params.%s;\n" % (n, n)
-    return ("template <int S>\n__global__ void __launch_bounds__(%d) %s__gpu(%s, double *rk_yn, double *rk_rt, "
+    return ("template <int S>\n__global__ void __launch_bounds__(%s) %s__gpu(%s, double *rk_yn, double *rk_rt, "
             "double *rk_out, const double dt) {\n%s%s}\n" %
-            (lp.MAX_THREADS, name, args_decl, unpack, "".join("  " + l + "\n" for l in kbody.splitlines())))
+            (lp.launch_bounds(), name, args_decl, unpack, "".join("  " + l + "\n" for l in kbody.splitlines())))
 
 
 _TOKEN = re.compile(r"[A-Za-z_][A-Za-z_0-9]*|\d+\.\d*(?:[eE][-+]?\d+)?|\d+|\S")
@@ -326,14 +326,16 @@ def count_body(body):
 
 
 def write_bundle(bundle_name, parts, meta):
-    """Write csrc/generated/<bundle>.hip and <bundle>.json."""
-    os.makedirs(GENDIR, exist_ok=True)
+    """Write csrc/generated/<bundle>.hip and <bundle>.json (or into $NRPY_GENDIR, a sibling
+    directory of csrc/generated, when generating variants for tuning)."""
+    gendir = os.environ.get("NRPY_GENDIR", GENDIR)
+    os.makedirs(gendir, exist_ok=True)
     src = "// Generated by nrpytutorial_amd.codegen; do not edit.\n" + "\n".join(parts)
-    with open(os.path.join(GENDIR, bundle_name + ".hip"), "w") as f:
+    with open(os.path.join(gendir, bundle_name + ".hip"), "w") as f:
         f.write(src)
-    with open(os.path.join(GENDIR, bundle_name + ".json"), "w") as f:
+    with open(os.path.join(gendir, bundle_name + ".json"), "w") as f:
         json.dump(meta, f, indent=1)
-    return os.path.join(GENDIR, bundle_name + ".hip")
+    return os.path.join(gendir, bundle_name + ".hip")
 
 
 def gf_defines_and_meta():

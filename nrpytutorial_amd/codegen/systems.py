@@ -116,8 +116,12 @@ def _fused_launcher(name, args_sig, call_args, loopopts):
             % (name, args_sig, kid, bd.lp.BLOCK_X, bd.lp.BLOCK_Y, bd.lp.BLOCK_Z, n[0], n[1], n[2], cases))
 
 
-# Output groups of the BSSN RHS, one kernel each (fewer live values per kernel).
-DEFAULT_RHS_GROUPS = "aDD,hDD,alpha,cf,trK|lambdaU,vetU,betU"
+# Output groups of the BSSN RHS, one kernel each: fewer live values per kernel. Measured on
+# MI355X at 192^3 (fused RHS + RK4 update, one substep): one kernel 28 ms; two groups
+# (aDD,hDD,alpha,cf,trK | lambdaU,vetU,betU) 10.2 ms; these three 8.5 ms. Grouping
+# lambdaU with betU avoids evaluating the Lambdabar RHS twice (the Gamma-driver B^i RHS
+# contains it).
+DEFAULT_RHS_GROUPS = "aDD,trK|hDD,alpha,cf|lambdaU,vetU,betU"
 
 
 def rhs_groups(names):
@@ -136,6 +140,22 @@ def rhs_groups(names):
     covered = [n for g in groups for n in g]
     if sorted(covered) != sorted(names):
         raise ValueError("RHS groups do not partition the outputs: " + spec)
+    return groups
+
+
+DEFAULT_RICCI_GROUPS = "RbarDD00,RbarDD01,RbarDD02|RbarDD11,RbarDD12,RbarDD22"
+
+
+def ricci_groups(names):
+    """Partition the RbarDD components by NRPY_BSSN_RICCI_GROUPS (``|``-separated groups of
+    comma-separated component names; ``all`` = one kernel)."""
+    import os
+    spec = os.environ.get("NRPY_BSSN_RICCI_GROUPS", DEFAULT_RICCI_GROUPS)
+    if spec == "all":
+        return [list(names)]
+    groups = [[n for n in g.split(",") if n] for g in spec.split("|")]
+    if sorted(n for g in groups for n in g) != sorted(names):
+        raise ValueError("Ricci groups do not partition the components: " + spec)
     return groups
 
 
@@ -318,14 +338,26 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
               "const double *auxevol_gfs, const double *in_gfs, double *y_n, double *rt, double *out, "
               "const double dt, hipStream_t stream) {\n  int rc;\n%s  return NRPY_OK;\n}\n" % calls)
     t0 = time.time()
-    ricci_body = fin.FD_outputC("returnstring",
-                                [lhrh(lhs=gri.gfaccess("auxevol_gfs", "RbarDD%d%d" % (i, j)), rhs=RbarDD[i][j])
-                                 for i in range(3) for j in range(i, 3)], params="outCverbose=False")
-    print("(BENCH) Ricci codegen %.1fs" % (time.time() - t0), flush=True)
-    ricci_src = oc.outCfunction(outfile="returnstring", desc="Evaluate the Ricci tensor", name="Ricci_eval",
-                                params="rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
-                                       "const REAL *restrict in_gfs,REAL *restrict auxevol_gfs",
-                                body=ricci_body, loopopts="InteriorPoints,Enable_rfm_precompute")
+    ric_names = ["RbarDD%d%d" % (i, j) for i in range(3) for j in range(i, 3)]
+    ric_exprs = {"RbarDD%d%d" % (i, j): RbarDD[i][j] for i in range(3) for j in range(i, 3)}
+    ric_groups = ricci_groups(ric_names)
+    ricci_params = ("rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
+                    "const REAL *restrict in_gfs,REAL *restrict auxevol_gfs")
+    ricci_src, ricci_costs = "", []
+    for k, grp in enumerate(ric_groups):
+        body_k = fin.FD_outputC("returnstring", [lhrh(lhs=gri.gfaccess("auxevol_gfs", n), rhs=ric_exprs[n])
+                                                 for n in grp], params="outCverbose=False")
+        ricci_costs.append(bd.count_body(body_k))
+        ricci_src += oc.outCfunction(outfile="returnstring", desc="Evaluate the Ricci tensor components %s"
+                                     % ", ".join(grp), name="Ricci_eval_g%d" % k, params=ricci_params, body=body_k,
+                                     loopopts="InteriorPoints,Enable_rfm_precompute")
+    calls = "".join("  if ((rc = Ricci_eval_g%d(rfmstruct, params, in_gfs, auxevol_gfs, stream))) return rc;\n" % k
+                    for k in range(len(ric_groups)))
+    ricci_src += ("/* Evaluate the Ricci tensor (%d kernels, one per group of components). */\n"
+                  'extern "C" int Ricci_eval(const rfm_struct *rfmstruct, const paramstruct *params, '
+                  "const double *in_gfs, double *auxevol_gfs, hipStream_t stream) {\n"
+                  "  int rc;\n%s  return NRPY_OK;\n}\n" % (len(ric_groups), calls))
+    print("(BENCH) Ricci codegen %.1fs (%d groups)" % (time.time() - t0, len(ric_groups)), flush=True)
     t0 = time.time()
     h_body = fin.FD_outputC("returnstring", lhrh(lhs=gri.gfaccess("aux_gfs", "H"), rhs=bssncon.H),
                             params="outCverbose=False")
@@ -354,9 +386,10 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
              ricci_src, rhs_src, fused, h_src, detg_src, id_src,
              _mol_glue(True, True, None, "rfmstruct, params, auxevol_gfs, IN")]
     meta["rhs_groups"] = groups
+    meta["ricci_groups"] = ric_groups
     meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
     meta["cost"] = {"rhs_eval": {"flops": sum(c["flops"] for c in group_costs), "groups": group_costs},
-                    "Ricci_eval": bd.count_body(ricci_body),
+                    "Ricci_eval": {"flops": sum(c["flops"] for c in ricci_costs), "groups": ricci_costs},
                     "Hamiltonian_constraint": bd.count_body(h_body),
                     "enforce_detgammabar_constraint": bd.count_body(detg_body)}
     meta.update({"bundle": bundle_name, "CoordSystem": CoordSystem, "symmetry_axes": symmetry_axes,

@@ -25,6 +25,14 @@ BLOCK_Y = 1
 BLOCK_Z = 4
 # every kernel is compiled for blocks of up to this many threads (the tunable shapes obey it)
 MAX_THREADS = 256
+
+
+def launch_bounds():
+    """``__launch_bounds__`` argument list. NRPY_MIN_WAVES_PER_EU (tuning) asks the
+    compiler for at least that many waves per SIMD, capping registers per lane."""
+    import os
+    w = os.environ.get("NRPY_MIN_WAVES_PER_EU", "")
+    return "%d, %d" % (MAX_THREADS, int(w)) if w else "%d" % MAX_THREADS
 # kernel name -> id into the runtime's tunable block-shape table
 KERNEL_IDS = {}
 
@@ -171,8 +179,8 @@ def kernel_and_launcher(name, desc, args, preloop, body, loopopts, postloop):
         if not any(n == x for x, _ in needs):
             unpack += "  const int %s = params.%s;\n" % (n, n)
     kname = name + "__gpu"
-    kernel = ("/* %s */\n__global__ void __launch_bounds__(%d) %s(%s) {\n%s%s}\n" %
-              (desc, MAX_THREADS, kname, ", ".join(kargs), unpack,
+    kernel = ("/* %s */\n__global__ void __launch_bounds__(%s) %s(%s) {\n%s%s}\n" %
+              (desc, launch_bounds(), kname, ", ".join(kargs), unpack,
                "".join("  " + l + "\n" for l in full.splitlines())))
     rng = "interior" if o["range"] == "interior" else "all"
     if rng == "interior":
