@@ -34,7 +34,8 @@ from . import bundle as bd
 
 CURVI_SCALARWAVE = {"scalarwave_sinhsphericalv2": "SinhSphericalv2",
                     "scalarwave_sinhcylindricalv2": "SinhCylindricalv2"}
-BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2") + tuple(CURVI_SCALARWAVE)
+BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2", "maxwell_cartesian") + \
+    tuple(CURVI_SCALARWAVE)
 
 
 def _mol_glue(has_ricci, has_detg, rhs_args_decl, rhs_call_args):
@@ -211,6 +212,53 @@ def scalarwave_cartesian():
     meta.update({"bundle": "scalarwave_cartesian", "CoordSystem": "Cartesian", "rfm_tables": [],
                  "kernels": ["rhs_eval", "initial_data"], "has_ricci": False, "has_detg": False})
     return bd.write_bundle("scalarwave_cartesian", parts, meta)
+
+
+def maxwell_cartesian():
+    """Vacuum Maxwell System II in Cartesian coordinates
+    (Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Cartesian.ipynb):
+    rhs_eval from VacuumMaxwellRHSs, toroidal-dipole initial data, the constraints
+    DivE and G as a diagnostic kernel, quadratic-extrapolation outer boundaries."""
+    from ..Maxwell import InitialData as mwid, VacuumMaxwell_Flat_Evol_Cartesian as mw
+    par.set_parval_from_str("grid::DIM", 3)
+    par.set_parval_from_str("finite_difference::FD_CENTDERIVS_ORDER", 4)
+    par.set_parval_from_str("reference_metric::CoordSystem", "Cartesian")
+    par.set_parval_from_str("Maxwell.InitialData::System_to_use", "System_II")
+    rfm.reference_metric()
+    mwid.InitialData()
+    id_rhss = [mwid.AidU[0], mwid.AidU[1], mwid.AidU[2], mwid.EidU[0], mwid.EidU[1], mwid.EidU[2],
+               mwid.Gamma_ID, mwid.psi_ID]
+    mw.VacuumMaxwellRHSs()
+    names = ["AU0", "AU1", "AU2", "EU0", "EU1", "EU2", "Gamma", "psi"]
+    rhss = list(mw.ArhsU) + list(mw.ErhsU) + [mw.Gamma_rhs, mw.psi_rhs]
+    exprs = [lhrh(lhs=gri.gfaccess("rhs_gfs", n), rhs=e) for n, e in zip(names, rhss)]
+    body = fin.FD_outputC("returnstring", exprs, params="outCverbose=False")
+    rhs_src = oc.outCfunction(outfile="returnstring", desc="Evaluate the vacuum Maxwell System II RHSs",
+                              name="rhs_eval", params="const paramstruct *restrict params,"
+                                                      "const REAL *restrict in_gfs,REAL *restrict rhs_gfs",
+                              body=body, loopopts="InteriorPoints")
+    con_body = fin.FD_outputC("returnstring", [lhrh(lhs=gri.gfaccess("aux_gfs", "DivE"), rhs=mw.C),
+                                               lhrh(lhs=gri.gfaccess("aux_gfs", "G"), rhs=mw.G)],
+                              params="outCverbose=False")
+    con_src = oc.outCfunction(outfile="returnstring", desc="Evaluate the constraints div E and G = Gamma - div A",
+                              name="Maxwell_constraints", params="const paramstruct *restrict params,"
+                                                                 "const REAL *restrict in_gfs,REAL *restrict aux_gfs",
+                              body=con_body, loopopts="InteriorPoints")
+    args_decl = "const paramstruct params, const double *__restrict__ in_gfs"
+    fused = bd.fused_rk4_kernel("rhs_eval_rk4_substep", args_decl, "", body, loopopts="InteriorPoints")
+    fused += _fused_launcher("rhs_eval_rk4_substep", "const rfm_struct *rfmstruct, const paramstruct *params, "
+                             "const double *auxevol_gfs, const double *in_gfs", "*params, in_gfs", "InteriorPoints")
+    id_src = _initial_data_kernel(id_rhss, names)
+    coords = bd.coord_host_functions("Cartesian")
+    pre, meta = _header(3)
+    parts = [pre, coords, '#include "../nrpy_runtime.h"', bd.rfm_struct_code(), rhs_src, con_src, fused, id_src,
+             _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN")]
+    meta["cost"] = {"rhs_eval": bd.count_body(body)}
+    meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
+    meta.update({"bundle": "maxwell_cartesian", "family": "maxwell", "CoordSystem": "Cartesian", "rfm_tables": [],
+                 "kernels": ["rhs_eval", "Maxwell_constraints", "initial_data"], "has_ricci": False,
+                 "has_detg": False})
+    return bd.write_bundle("maxwell_cartesian", parts, meta)
 
 
 def scalarwave_curvilinear(CoordSystem, bundle_name, NGHOSTS=3):
@@ -409,8 +457,7 @@ def generate(name):
     if name in CURVI_SCALARWAVE:
         return scalarwave_curvilinear(CURVI_SCALARWAVE[name], name)
     if name == "maxwell_cartesian":
-        from . import maxwell
-        return maxwell.maxwell_cartesian()
+        return maxwell_cartesian()
     raise ValueError("unknown bundle " + name)
 
 

@@ -307,6 +307,32 @@ def rfm_precompute_tables():
     return out
 
 
+def compute_Jacobian_and_inverseJacobian_tofrom_Cartesian():
+    """(d xCart^i / d xx^j, its inverse d xx^i / d xCart^j) for the current
+    coordinate system (reference_metric.py:904-911)."""
+    J = [[sp.diff(xx_to_Cart[i], xx[j]) for j in range(3)] for i in range(3)]
+    Jinv, _ = ixp.generic_matrix_inverter3x3(J)
+    return J, Jinv
+
+
+def basis_transform_vectorU_from_rfmbasis_to_Cartesian(Jac_dUCart_dDrfmUD, src_vectorU):
+    return [sum(Jac_dUCart_dDrfmUD[i][l] * src_vectorU[l] for l in range(3)) for i in range(3)]
+
+
+def basis_transform_vectorU_from_Cartesian_to_rfmbasis(Jac_dUrfm_dDCartUD, Cart_src_vectorU):
+    return [sum(Jac_dUrfm_dDCartUD[i][l] * Cart_src_vectorU[l] for l in range(3)) for i in range(3)]
+
+
+def basis_transform_tensorDD_from_rfmbasis_to_Cartesian(Jac_dUrfm_dDCartUD, src_tensorDD):
+    return [[sum(Jac_dUrfm_dDCartUD[l][i] * Jac_dUrfm_dDCartUD[m][j] * src_tensorDD[l][m]
+                 for l in range(3) for m in range(3)) for j in range(3)] for i in range(3)]
+
+
+def basis_transform_tensorDD_from_Cartesian_to_rfmbasis(Jac_dUCart_dDrfmUD, Cart_src_tensorDD):
+    return [[sum(Jac_dUCart_dDrfmUD[l][i] * Jac_dUCart_dDrfmUD[m][j] * Cart_src_tensorDD[l][m]
+                 for l in range(3) for m in range(3)) for j in range(3)] for i in range(3)]
+
+
 def get_EigenCoord():
     """Coordinate family used to build boundary-condition maps (reference_metric.py:946-952)."""
     cs = par.parval_from_str("reference_metric::CoordSystem")

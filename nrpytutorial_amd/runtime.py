@@ -80,7 +80,7 @@ KERNEL_PROTOTYPES = {
     "bssn": {"rhs_eval": [_VP, _VP, _VP, _VP, _VP, _VP], "Ricci_eval": [_VP, _VP, _VP, _VP, _VP],
              "Hamiltonian_constraint": [_VP, _VP, _VP, _VP, _VP],
              "enforce_detgammabar_constraint": [_VP, _VP, _VP, _VP]},
-    "maxwell": {"rhs_eval": [_VP, _VP, _VP, _VP]},
+    "maxwell": {"rhs_eval": [_VP, _VP, _VP, _VP], "Maxwell_constraints": [_VP, _VP, _VP, _VP]},
 }
 
 

@@ -41,6 +41,8 @@ class Oracle:
         L.oracle_find_timestep.argtypes = [_VP, _DPP, _D]
         L.oracle_find_timestep.restype = _D
         L.oracle_xx_to_Cart.argtypes = [_VP, _DPP, _I, _I, _I, _VP]
+        if hasattr(L, "oracle_Maxwell_constraints"):
+            L.oracle_Maxwell_constraints.argtypes = [_VP, _DPP, _VP, _VP]
         if hasattr(L, "oracle_Hamiltonian_constraint"):
             L.oracle_Hamiltonian_constraint.argtypes = [_VP, _DPP, _VP, _VP]
         self.NGHOSTS = L.oracle_NGHOSTS()
@@ -120,6 +122,12 @@ class OracleGrid:
         self.o.lib.oracle_Hamiltonian_constraint(ctypes.byref(self.p), self.ptrs(self.xx), y.ctypes.data,
                                                  aux.ctypes.data)
         return aux[0]
+
+    def maxwell_constraints(self, y):
+        aux = self.zeros(2)
+        self.o.lib.oracle_Maxwell_constraints(ctypes.byref(self.p), self.ptrs(self.xx), y.ctypes.data,
+                                              aux.ctypes.data)
+        return aux
 
     def enforce_detg(self, y):
         self.o.lib.oracle_enforce_detgammabar_constraint(ctypes.byref(self.p), self.ptrs(self.xx), y.ctypes.data)

@@ -18,7 +18,8 @@ INC = os.path.join(ROOT, "include")
 
 FAMILY_HEADER = {"scalarwave": "nrpy_amd_scalarwave.h",
                  "scalarwave_curvilinear": "nrpy_amd_scalarwave_curvilinear.h",
-                 "bssn": "nrpy_amd_bssn.h"}
+                 "bssn": "nrpy_amd_bssn.h",
+                 "maxwell": "nrpy_amd_maxwell.h"}
 
 
 def declared(header):

@@ -87,3 +87,33 @@ def test_sinhsphericalv2_fourth_order_convergence():
     r1 = np.log(errs[0] / errs[1]) / np.log(12 / 8)
     r2 = np.log(errs[1] / errs[2]) / np.log(16 / 12)
     assert 3.4 < r1 < 4.6 and 3.4 < r2 < 4.6, (errs, r1, r2)
+
+
+MAXWELL_BOX = dict(xmin=-4.0, xmax=4.0, ymin=-4.0, ymax=4.0, zmin=-4.0, zmax=4.0)
+
+
+def test_maxwell_fourth_order_convergence():
+    """Vacuum Maxwell System II, toroidal dipole (an exact solution for all t,
+    Maxwell/InitialData.py), Cartesian [-4,4]^3, evolved to t=0.5 with CFL 0.5 at
+    24^3, 32^3, 48^3: the state error and the constraint div E both fall at
+    fourth order."""
+    o = Oracle("maxwell_cartesian")
+    errs, cons = [], []
+    for n in (24, 32, 48):
+        g = OracleGrid(o, (n, n, n), MAXWELL_BOX)
+        y = g.zeros(8)
+        g.initial_data(y)
+        g.apply_bcs(y)
+        T = 0.5
+        dt = g.find_timestep(0.5)
+        nst = int(round(T / dt))
+        g.mol_step(y, T / nst, nst)
+        g2 = OracleGrid(o, (n, n, n), dict(MAXWELL_BOX, time=T))
+        ex = g2.zeros(8)
+        g2.initial_data(ex)
+        errs.append(np.max(np.abs(g.interior(y) - g.interior(ex))))
+        cons.append(np.max(np.abs(g.interior(g.maxwell_constraints(y)[0]))))
+    for e in (errs, cons):
+        r1 = np.log(e[0] / e[1]) / np.log(32 / 24)
+        r2 = np.log(e[1] / e[2]) / np.log(48 / 32)
+        assert 3.4 < r1 < 4.6 and 3.4 < r2 < 4.6, (e, r1, r2)

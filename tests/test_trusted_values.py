@@ -104,6 +104,22 @@ CASES += [
      "m", ['alphaCart', 'betaCartU', 'BCartU', 'gammaCartDD', 'KCartDD']),
 ]
 
+MAXWELL_SYS = "from nrpytutorial_amd.Maxwell import InitialData\npar.set_parval_from_str('Maxwell.InitialData::System_to_use', '%s')\n"
+CASES += [
+    ("VacuumMaxwell_Flat_Evol_Cartesian_System_I__VacuumMaxwellRHSs__globals",
+     MAXWELL_SYS % "System_I" + "from nrpytutorial_amd.Maxwell import VacuumMaxwell_Flat_Evol_Cartesian as m\n"
+     "m.VacuumMaxwellRHSs()\n", "m", ['ArhsU', 'ErhsU', 'C', 'psi_rhs']),
+    ("VacuumMaxwell_Flat_Evol_Cartesian_System_II__VacuumMaxwellRHSs__globals",
+     MAXWELL_SYS % "System_II" + "from nrpytutorial_amd.Maxwell import VacuumMaxwell_Flat_Evol_Cartesian as m\n"
+     "m.VacuumMaxwellRHSs()\n", "m", ['ArhsU', 'ErhsU', 'C', 'psi_rhs', 'Gamma_rhs', 'G']),
+    ("VacuumMaxwell_Flat_Evol_Curvilinear_rescaled__VacuumMaxwellRHSs_rescaled__globals",
+     MAXWELL_SYS % "System_II" + "from nrpytutorial_amd.Maxwell import VacuumMaxwell_Flat_Evol_Curvilinear_rescaled"
+     " as m\nm.VacuumMaxwellRHSs_rescaled()\n", "m",
+     ['erhsU', 'arhsU', 'psi_rhs', 'Gamma_rhs', 'C', 'G', 'EU_Cart', 'AU_Cart']),
+    ("InitialData_System_II__InitialData__globals",
+     MAXWELL_SYS % "System_II" + "m = InitialData\nm.InitialData()\n", "m", ['AidU', 'EidU', 'psi_ID', 'Gamma_ID']),
+]
+
 SLOW = {"BSSN_RHSs__BSSN_RHSs__globals", "gauge_RHSs__BSSN_gauge_RHSs__globals",
         "BSSN_constraints__BSSN_constraints__globals"}
 
@@ -134,11 +150,7 @@ def test_trusted_values(entry, setup, mod, names):
     assert failures == [], "\n".join(failures[:20])
 
 
-def test_every_fixture_entry_is_covered_or_listed():
-    """Entries in the fixture without a case here are the Maxwell ones, which
-    are tested with the Maxwell system."""
+def test_every_fixture_entry_is_covered():
     with open(os.path.join(HERE, "golden", "trusted_values.json")) as f:
         d = json.load(f)
-    covered = {c[0] for c in CASES}
-    rest = sorted(set(d) - covered)
-    assert all("Maxwell" in e or "System_II" in e for e in rest), rest
+    assert sorted(set(d) - {c[0] for c in CASES}) == []
