@@ -138,6 +138,64 @@ def bench_bl(torch):
     return {"gp_per_s": 4.0 * 72 * 12 * 2 * 50 / el, "seconds": el, "H_L2": l2, "dt": dt}
 
 
+def bench_adjoint(torch, N=128, nsteps=200):
+    """Adjoint scalar wave: 128^3 on [-10,10]^3, 200 RK4 steps, J = 0.5 sum uu(T)^2 dx^3,
+    gradient w.r.t. 16 bump amplitudes through the checkpointed autograd evolution.
+    Reports forward-step and adjoint-step times (the adjoint step includes recomputing
+    its three stage states, and the checkpointed backward re-runs each forward step)."""
+    from nrpytutorial_amd.runtime import Bundle, Grid, Evolution
+    from nrpytutorial_amd.adjoint import RK4Adjoint, RK4Evolve
+    from nrpytutorial_amd.workloads import scalarwave_bump_parameters, scalarwave_bump_basis_torch
+    b = Bundle("scalarwave_cartesian")
+    g = Grid(b, (N, N, N), wavespeed=1.0)
+    ev = Evolution(g)
+    adj = RK4Adjoint(ev)
+    amps, cents = scalarwave_bump_parameters(3)
+    basis = scalarwave_bump_basis_torch(g.xx_dev, cents)
+    dt = 0.5 * g.params.dxx0
+    dV = g.params.dxx0 * g.params.dxx1 * g.params.dxx2
+
+    def loss(a):
+        uu0 = 2.0 + (a[:, None, None, None] * basis).sum(0)
+        yT = RK4Evolve().apply(torch.stack([uu0, torch.zeros_like(uu0)]), ev, adj, dt, nsteps)
+        return 0.5 * (g.interior(yT[0]) ** 2).sum() * dV
+
+    a = torch.tensor(amps, dtype=torch.float64, device="cuda", requires_grad=True)
+    loss(a).backward()   # warm-up (graph capture, allocations)
+    torch.cuda.synchronize()
+    a = torch.tensor(amps, dtype=torch.float64, device="cuda", requires_grad=True)
+    t0 = time.perf_counter()
+    J = loss(a)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    # one forward step and one adjoint step in isolation
+    reps = 20
+    ev.step(dt, 1)
+    y = ev.y_n.clone()
+    lam = torch.zeros_like(y)
+    g.interior(lam).normal_()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    ev.step(dt, reps)
+    e.record()
+    e.synchronize()
+    t_fwd = s.elapsed_time(e) / 1000.0 / reps
+    s.record()
+    for _ in range(reps):
+        adj.step_back(y, lam, dt)
+    e.record()
+    e.synchronize()
+    t_adj = s.elapsed_time(e) / 1000.0 / reps
+    return {"grid": [N, N, N], "steps": nsteps, "forward_s": t1 - t0, "backward_s": t2 - t1,
+            "forward_step_ms": 1e3 * t_fwd, "adjoint_step_ms": 1e3 * t_adj, "adjoint_over_forward": t_adj / t_fwd,
+            "backward_over_forward": (t2 - t1) / (t1 - t0),
+            "gp_per_s_adjoint_steps": 4.0 * N ** 3 / t_adj, "J": float(J.detach())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +204,7 @@ def main():
     ap.add_argument("--n", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bl", action="store_true", help="skip the Brill-Lindquist 72x12x2 line")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint scalar-wave line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (scale for the HBM byte counters)")
     ap.add_argument("--cpu-baseline-worker", nargs=3)
@@ -242,6 +301,12 @@ def main():
         bl = bench_bl(torch)
         log("BL 72x12x2: 50 steps in %.4f s -> %.3e gp/s, H-L2 %.6e" % (bl["seconds"], bl["gp_per_s"], bl["H_L2"]))
 
+    adjoint = None
+    if not a.no_adjoint:
+        adjoint = bench_adjoint(torch)
+        log("adjoint scalar wave 128^3: forward step %.3f ms, adjoint step %.3f ms (%.2fx); 200-step loss %.3f s, "
+            "gradient %.3f s" % (adjoint["forward_step_ms"], adjoint["adjoint_step_ms"], adjoint["adjoint_over_forward"],
+                                 adjoint["forward_s"], adjoint["backward_s"]))
     cpu = None
     if not a.no_cpu:
         try:
@@ -273,7 +338,7 @@ def main():
                                 **{"rhs_eval_rk4_substep_g%d" % k: 1e3 * t for k, t in enumerate(t_group)}},
                   "fused_substep_fp64_TFLOPs": fused_flops * npts / t_fused / 1e12,
                   "fused_flops_per_point": fused_flops, "ricci_flops_per_point": cost["Ricci_eval"]["flops"],
-                  "rhs_groups": groups, "bl_72x12x2_50steps": bl},
+                  "rhs_groups": groups, "bl_72x12x2_50steps": bl, "adjoint_scalarwave_128": adjoint},
     }
     print(json.dumps(out))
     return 0

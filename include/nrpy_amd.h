@@ -146,6 +146,21 @@ int rhs_eval_rk4_substep(const int S, const rfm_struct *rfmstruct, const paramst
                          const double *auxevol_gfs, const double *in_gfs, double *y_n, double *rt, double *out,
                          const double dt, hipStream_t stream);
 
+/* ---- adjoint -------------------------------------------------------------- */
+
+/* Transpose of the ghost fill (no counterpart in the reference, which has no
+ * adjoint): adds to every interior point the cotangents of the ghost points whose
+ * values were computed from it (with the fill's coefficients and parity signs), then
+ * zeroes the ghost cotangents. cot_gfs is a device array of NUM_GFS gridfunctions. */
+int bc_apply_T(const paramstruct *params, const bc_struct *bc, const int NUM_GFS, const int8_t *gfs_parity,
+               double *cot_gfs, hipStream_t stream);
+
+/* Device vector helpers used by the adjoint step: out = a*x + b*y (y may be NULL),
+ * and out = v, over n doubles. Kernels, not memsets, so they can be graph-captured. */
+int nrpy_lincomb(size_t n, double *out, double a, const double *x, double b, const double *y,
+                 hipStream_t stream);
+int nrpy_fill(size_t n, double *out, double v, hipStream_t stream);
+
 /* ---- diagnostics and tuning ---------------------------------------------- */
 
 /* Sum over interior points of gfs[gf]^2 into *result (device), deterministic

@@ -20,6 +20,23 @@ int Maxwell_constraints(const paramstruct *params, const double *in_gfs, double 
 /* Toroidal dipole data (Maxwell/InitialData.py:25-113) at all points. */
 int initial_data(const paramstruct *params, double *xx[3], double *in_gfs, hipStream_t stream);
 
+/* Vector-Jacobian product of rhs_eval (no counterpart in the reference): for the
+ * cotangent cot_rhs_gfs of the RHS at interior points, the cotangent of every input
+ * gridfunction at every point, ghost points included (fold those with bc_apply_T).
+ * vjp_G: scratch of <bundle>.json "vjp_slots" gridfunctions (one per pointwise slot:
+ * gridfunction value or stencil). auxevol_gfs and cot_auxevol_gfs may be NULL. */
+int rhs_eval_vjp(const paramstruct *params, const double *auxevol_gfs, const double *in_gfs,
+                 const double *cot_rhs_gfs, double *vjp_G, double *cot_in_gfs, double *cot_auxevol_gfs,
+                 hipStream_t stream);
+
+/* Adjoint of one RK4 step (MoL_step_forward_in_time): lambda_gfs holds the cotangent of
+ * y_{n+1} on entry and that of y_n on exit; y_n_gfs is the state at the start of the step.
+ * U1..U3, rt, acc, kbar, w: scratch states; vjp_G as for rhs_eval_vjp. */
+int MoL_step_adjoint(const rfm_struct *rfmstruct, const paramstruct *params, const bc_struct *bcstruct,
+                     const double *y_n_gfs, double *lambda_gfs, double *U1, double *U2, double *U3, double *rt,
+                     double *acc, double *kbar, double *w, double *vjp_G, double *auxevol_gfs, const double dt,
+                     hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

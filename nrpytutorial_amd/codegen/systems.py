@@ -31,6 +31,7 @@ from .. import finite_difference as fin
 from .. import outputC as oc
 from ..outputC import lhrh
 from . import bundle as bd
+from . import adjoint as adj
 
 CURVI_SCALARWAVE = {"scalarwave_sinhsphericalv2": "SinhSphericalv2",
                     "scalarwave_sinhcylindricalv2": "SinhCylindricalv2"}
@@ -95,6 +96,65 @@ def _mol_glue(has_ricci, has_detg, rhs_args_decl, rhs_call_args):
         "  return NRPY_OK;\n}\n"
         'extern "C" void MoL_graph_free(MoL_graph *g) {\n'
         "  if (!g) return;\n  (void)hipGraphExecDestroy(g->exec);\n  (void)hipGraphDestroy(g->graph);\n  delete g;\n}\n")
+
+
+def _mol_adjoint_glue(has_rfm_vjp):
+    """MoL_step_adjoint: the adjoint of one RK4 step (diagonal schedule, ghost fill after
+    every substep) for systems whose step is [RHS + update -> ghost fill] per substep.
+
+    With u the interior of y_n and E the ghost fill, the step is
+      k1 = R(E u), u1 = u + dt/2 k1, k2 = R(E u1), u2 = u + dt/2 k2, k3 = R(E u2),
+      u3 = u + dt k3, k4 = R(E u3), u_new = u + dt/6 (k1 + 2 k2 + 2 k3 + k4),
+    so for the cotangent L of u_new, with w_s = E^T R'(E u_s)^T kbar_s:
+      kbar4 = dt/6 L, kbar3 = dt/3 L + dt w3, kbar2 = dt/3 L + dt/2 w2, kbar1 = dt/6 L + dt/2 w1,
+      L_n = L + w3 + w2 + w1 + w0.
+    The stage states are recomputed from y_n with the forward substep kernels."""
+    rfm_arg = "rfmstruct, " if has_rfm_vjp else ""
+    vjp = ("  if ((rc = nrpy_fill(n, w, 0.0, stream))) return rc;\n"
+           "  if ((rc = rhs_eval_vjp(" + rfm_arg + "params, auxevol_gfs, STATE, kbar, vjp_G, w, nullptr, stream))) "
+           "return rc;\n"
+           "  if ((rc = bc_apply_T(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, w, stream))) return rc;\n")
+    return (
+        "// Adjoint of one RK4 step. On entry lambda_gfs holds the cotangent of y_{n+1} (interior points;\n"
+        "// ghost cotangents zero), y_n_gfs the state at the start of the step with its ghosts filled. On\n"
+        "// exit lambda_gfs holds the cotangent of y_n. U1, U2, U3, rt, acc, kbar, w are caller-allocated\n"
+        "// scratch arrays of NUM_EVOL_GFS gridfunctions; vjp_G has the rhs_eval_vjp slot count.\n"
+        'extern "C" int MoL_step_adjoint(const rfm_struct *rfmstruct, const paramstruct *params,\n'
+        "    const bc_struct *bcstruct, const double *y_n_gfs, double *lambda_gfs, double *U1, double *U2,\n"
+        "    double *U3, double *rt, double *acc, double *kbar, double *w, double *vjp_G, double *auxevol_gfs,\n"
+        "    const double dt, hipStream_t stream) {\n"
+        "  if (!rfmstruct || !params || !bcstruct || !y_n_gfs || !lambda_gfs || !U1 || !U2 || !U3 || !rt ||\n"
+        "      !acc || !kbar || !w || !vjp_G) return NRPY_ERR_NULL;\n"
+        "  int rc;\n"
+        "  const size_t n = (size_t)NUM_EVOL_GFS * params->Nxx_plus_2NGHOSTS0 * params->Nxx_plus_2NGHOSTS1 *\n"
+        "                   params->Nxx_plus_2NGHOSTS2;\n"
+        "  double *yn = (double *)y_n_gfs;  // substeps 0-2 read y_n only\n"
+        "  if ((rc = rhs_eval_rk4_substep(0, rfmstruct, params, auxevol_gfs, yn, yn, rt, U1, dt, stream))) return rc;\n"
+        "  if ((rc = apply_bcs_curvilinear_composed(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, U1, stream))) "
+        "return rc;\n"
+        "  if ((rc = rhs_eval_rk4_substep(1, rfmstruct, params, auxevol_gfs, U1, yn, rt, U2, dt, stream))) return rc;\n"
+        "  if ((rc = apply_bcs_curvilinear_composed(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, U2, stream))) "
+        "return rc;\n"
+        "  if ((rc = rhs_eval_rk4_substep(2, rfmstruct, params, auxevol_gfs, U2, yn, rt, U3, dt, stream))) return rc;\n"
+        "  if ((rc = apply_bcs_curvilinear_composed(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, U3, stream))) "
+        "return rc;\n"
+        "  /* stage 4 */\n"
+        "  if ((rc = nrpy_lincomb(n, kbar, dt / 6.0, lambda_gfs, 0.0, nullptr, stream))) return rc;\n"
+        + vjp.replace("STATE", "U3") +
+        "  if ((rc = nrpy_lincomb(n, acc, 1.0, lambda_gfs, 1.0, w, stream))) return rc;\n"
+        "  /* stage 3 */\n"
+        "  if ((rc = nrpy_lincomb(n, kbar, dt / 3.0, lambda_gfs, dt, w, stream))) return rc;\n"
+        + vjp.replace("STATE", "U2") +
+        "  if ((rc = nrpy_lincomb(n, acc, 1.0, acc, 1.0, w, stream))) return rc;\n"
+        "  /* stage 2 */\n"
+        "  if ((rc = nrpy_lincomb(n, kbar, dt / 3.0, lambda_gfs, dt / 2.0, w, stream))) return rc;\n"
+        + vjp.replace("STATE", "U1") +
+        "  if ((rc = nrpy_lincomb(n, acc, 1.0, acc, 1.0, w, stream))) return rc;\n"
+        "  /* stage 1 */\n"
+        "  if ((rc = nrpy_lincomb(n, kbar, dt / 6.0, lambda_gfs, dt / 2.0, w, stream))) return rc;\n"
+        + vjp.replace("STATE", "y_n_gfs") +
+        "  return nrpy_lincomb(n, lambda_gfs, 1.0, acc, 1.0, w, stream);\n"
+        "}\n")
 
 
 def _fused_launcher(name, args_sig, call_args, loopopts):
@@ -204,10 +264,12 @@ def scalarwave_cartesian():
     fused += _fused_launcher("rhs_eval_rk4_substep", "const rfm_struct *rfmstruct, const paramstruct *params, "
                              "const double *auxevol_gfs, const double *in_gfs", "*params, in_gfs", "InteriorPoints")
     id_src = _initial_data_kernel([swid.uu_ID, swid.vv_ID], ["uu", "vv"])
+    vjp_src, slots, _ = adj.vjp_kernels("rhs_eval", exprs, "", "InteriorPoints", "")
     pre, meta = _header(3)
     parts = [pre, bd.coord_host_functions("Cartesian"), '#include "../nrpy_runtime.h"', bd.rfm_struct_code(),
-             rhs_src, fused, id_src,
-             _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN")]
+             rhs_src, fused, id_src, vjp_src,
+             _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN"), _mol_adjoint_glue(False)]
+    meta["vjp_slots"] = [sl[0] for sl in slots]
     meta["cost"] = {"rhs_eval": bd.count_body(body)}
     meta.update({"bundle": "scalarwave_cartesian", "CoordSystem": "Cartesian", "rfm_tables": [],
                  "kernels": ["rhs_eval", "initial_data"], "has_ricci": False, "has_detg": False})
@@ -251,8 +313,10 @@ def maxwell_cartesian():
     id_src = _initial_data_kernel(id_rhss, names)
     coords = bd.coord_host_functions("Cartesian")
     pre, meta = _header(3)
+    vjp_src, slots, _ = adj.vjp_kernels("rhs_eval", exprs, "", "InteriorPoints", "")
     parts = [pre, coords, '#include "../nrpy_runtime.h"', bd.rfm_struct_code(), rhs_src, con_src, fused, id_src,
-             _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN")]
+             vjp_src, _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN"), _mol_adjoint_glue(False)]
+    meta["vjp_slots"] = [sl[0] for sl in slots]
     meta["cost"] = {"rhs_eval": bd.count_body(body)}
     meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
     meta.update({"bundle": "maxwell_cartesian", "family": "maxwell", "CoordSystem": "Cartesian", "rfm_tables": [],

@@ -79,6 +79,15 @@ struct bc_struct {
   int *d_negmask = nullptr;        // bit t set: parity type t contributes with a minus sign
   std::vector<int> h_gdest, h_rowptr, h_src, h_negmask;
   std::vector<double> h_coeff;
+  // transpose of the composed map (CSR over interior source points), for bc_apply_T
+  int ntsrc = 0;
+  int *d_tsrc = nullptr;
+  int *d_trowptr = nullptr;
+  int *d_tghost = nullptr;
+  double *d_tcoeff = nullptr;
+  int *d_tnegmask = nullptr;
+  std::vector<int> h_tsrc, h_trowptr, h_tghost, h_tnegmask;
+  std::vector<double> h_tcoeff;
 };
 
 struct nrpy_gf_parity_args {
@@ -288,6 +297,31 @@ static int nrpy_build_composed_map(const paramstruct *params, bc_struct *bc) {
   }
   bc->nghost = (int)bc->h_gdest.size();
   bc->nterms = (int)bc->h_src.size();
+  // Transpose: for every interior source point, the ghost points whose value it enters.
+  {
+    std::vector<int> perm(bc->h_src.size()), termrow(bc->h_src.size());
+    for (int r = 0; r < bc->nghost; r++)
+      for (int k = bc->h_rowptr[r]; k < bc->h_rowptr[r + 1]; k++) termrow[k] = r;
+    for (size_t k = 0; k < perm.size(); k++) perm[k] = (int)k;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return bc->h_src[a] < bc->h_src[b]; });
+    bc->h_tsrc.clear();
+    bc->h_trowptr.assign(1, 0);
+    bc->h_tghost.clear();
+    bc->h_tcoeff.clear();
+    bc->h_tnegmask.clear();
+    for (size_t q = 0; q < perm.size(); q++) {
+      const int k = perm[q];
+      if (bc->h_tsrc.empty() || bc->h_tsrc.back() != bc->h_src[k]) {
+        if (!bc->h_tsrc.empty()) bc->h_trowptr.push_back((int)bc->h_tghost.size());
+        bc->h_tsrc.push_back(bc->h_src[k]);
+      }
+      bc->h_tghost.push_back(bc->h_gdest[termrow[k]]);
+      bc->h_tcoeff.push_back(bc->h_coeff[k]);
+      bc->h_tnegmask.push_back(bc->h_negmask[k]);
+    }
+    if (!bc->h_tsrc.empty()) bc->h_trowptr.push_back((int)bc->h_tghost.size());
+    bc->ntsrc = (int)bc->h_tsrc.size();
+  }
   return NRPY_OK;
 }
 
@@ -311,6 +345,11 @@ extern "C" void bcstruct_free(bc_struct *bc) {
   (void)hipFree(bc->d_src);
   (void)hipFree(bc->d_coeff);
   (void)hipFree(bc->d_negmask);
+  (void)hipFree(bc->d_tsrc);
+  (void)hipFree(bc->d_trowptr);
+  (void)hipFree(bc->d_tghost);
+  (void)hipFree(bc->d_tcoeff);
+  (void)hipFree(bc->d_tnegmask);
   delete bc;
 }
 
@@ -375,7 +414,12 @@ extern "C" int bcstruct_create(const paramstruct *params, double *xx[3], bc_stru
             nrpy_upload(&bc->d_rowptr, bc->h_rowptr.data(), bc->h_rowptr.size()) &&
             nrpy_upload(&bc->d_src, bc->h_src.data(), bc->h_src.size()) &&
             nrpy_upload(&bc->d_coeff, bc->h_coeff.data(), bc->h_coeff.size()) &&
-            nrpy_upload(&bc->d_negmask, bc->h_negmask.data(), bc->h_negmask.size());
+            nrpy_upload(&bc->d_negmask, bc->h_negmask.data(), bc->h_negmask.size()) &&
+            nrpy_upload(&bc->d_tsrc, bc->h_tsrc.data(), bc->h_tsrc.size()) &&
+            nrpy_upload(&bc->d_trowptr, bc->h_trowptr.data(), bc->h_trowptr.size()) &&
+            nrpy_upload(&bc->d_tghost, bc->h_tghost.data(), bc->h_tghost.size()) &&
+            nrpy_upload(&bc->d_tcoeff, bc->h_tcoeff.data(), bc->h_tcoeff.size()) &&
+            nrpy_upload(&bc->d_tnegmask, bc->h_tnegmask.data(), bc->h_tnegmask.size());
   if (!ok) {
     bcstruct_free(bc);
     return NRPY_ERR_ALLOC;
@@ -525,6 +569,96 @@ extern "C" int apply_bcs_curvilinear_composed(const paramstruct *params, const b
   hipLaunchKernelGGL(nrpy_bc_composed_kernel, dim3((unsigned)((work + B - 1) / B)), dim3(B), 0, stream,
                      bc->nghost, bc->d_gdest, bc->d_rowptr, bc->d_src, bc->d_coeff, bc->d_negmask, NUM_GFS, gfp,
                      Ntot, gfs);
+  return hipGetLastError() == hipSuccess ? NRPY_OK : NRPY_ERR_LAUNCH;
+}
+
+// ---------------------------------------------------------------------------
+// 3b. Transpose of the ghost fill, for adjoints. The ghost fill is linear:
+// ghost = sum_k c_k s_k src_k (s_k the parity sign of the gridfunction's type), so
+// its transpose adds, at every interior source point, sum_k c_k s_k cot(ghost_k),
+// and then the ghost cotangents are cleared. Sources are interior and ghosts are
+// not, so the gather reads and writes disjoint points; a second launch clears.
+
+__global__ void nrpy_bc_transpose_kernel(int ntsrc, const int *__restrict__ tsrc, const int *__restrict__ trowptr,
+                                         const int *__restrict__ tghost, const double *__restrict__ tcoeff,
+                                         const int *__restrict__ tnegmask, int num_gfs, nrpy_gf_parity_args gfp,
+                                         size_t Ntot, double *__restrict__ gfs) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)ntsrc * num_gfs) return;
+  const int gf = (int)(t / ntsrc);
+  const int r = (int)(t % ntsrc);
+  double *g = gfs + (size_t)gf * Ntot;
+  const int bit = 1 << gfp.p[gf];
+  double acc = 0.0;
+  for (int k = trowptr[r]; k < trowptr[r + 1]; k++) {
+    const double c = (tnegmask[k] & bit) ? -tcoeff[k] : tcoeff[k];
+    acc += c * g[tghost[k]];
+  }
+  g[tsrc[r]] += acc;
+}
+
+__global__ void nrpy_clear_ghosts_kernel(int nghost, const int *__restrict__ gdest, int num_gfs, size_t Ntot,
+                                         double *__restrict__ gfs) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)nghost * num_gfs) return;
+  gfs[(size_t)(t / nghost) * Ntot + gdest[t % nghost]] = 0.0;
+}
+
+// Transpose of apply_bcs_curvilinear(_composed): folds the cotangents held at ghost
+// points onto the interior points the ghost values were computed from, then zeroes
+// the ghost cotangents. No counterpart in the reference (it has no adjoint).
+extern "C" int bc_apply_T(const paramstruct *params, const bc_struct *bc, const int NUM_GFS,
+                          const int8_t *gfs_parity, double *cot_gfs, hipStream_t stream) {
+  nrpy_gf_parity_args gfp;
+  memset(&gfp, 0, sizeof(gfp));
+  int rc = nrpy_bc_check(params, bc, NUM_GFS, gfs_parity, cot_gfs, &gfp);
+  if (rc) return rc;
+  if (bc->nghost == 0) return NRPY_OK;
+  const size_t Ntot = (size_t)bc->N[0] * bc->N[1] * bc->N[2];
+  const int B = 256;
+  if (bc->ntsrc > 0) {
+    const size_t work = (size_t)bc->ntsrc * NUM_GFS;
+    hipLaunchKernelGGL(nrpy_bc_transpose_kernel, dim3((unsigned)((work + B - 1) / B)), dim3(B), 0, stream,
+                       bc->ntsrc, bc->d_tsrc, bc->d_trowptr, bc->d_tghost, bc->d_tcoeff, bc->d_tnegmask, NUM_GFS,
+                       gfp, Ntot, cot_gfs);
+  }
+  const size_t work = (size_t)bc->nghost * NUM_GFS;
+  hipLaunchKernelGGL(nrpy_clear_ghosts_kernel, dim3((unsigned)((work + B - 1) / B)), dim3(B), 0, stream,
+                     bc->nghost, bc->d_gdest, NUM_GFS, Ntot, cot_gfs);
+  return hipGetLastError() == hipSuccess ? NRPY_OK : NRPY_ERR_LAUNCH;
+}
+
+__global__ void nrpy_lincomb_kernel(size_t n, double *__restrict__ out, double a, const double *__restrict__ x,
+                                    double b, const double *__restrict__ y) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = y ? a * x[i] + b * y[i] : a * x[i];
+}
+
+__global__ void nrpy_fill_kernel(size_t n, double *__restrict__ out, double v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = v;
+}
+
+// out[i] = v for i < n (a kernel, not a memset, so it is safe inside captured graphs).
+extern "C" int nrpy_fill(size_t n, double *out, double v, hipStream_t stream) {
+  if (!out) return NRPY_ERR_NULL;
+  if (n == 0) return NRPY_OK;
+  const int B = 256;
+  size_t nb = (n + B - 1) / B;
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL(nrpy_fill_kernel, dim3((unsigned)nb), dim3(B), 0, stream, n, out, v);
+  return hipGetLastError() == hipSuccess ? NRPY_OK : NRPY_ERR_LAUNCH;
+}
+
+// out[i] = a*x[i] + b*y[i] for i < n (y may be NULL: out = a*x). out may alias x or y.
+extern "C" int nrpy_lincomb(size_t n, double *out, double a, const double *x, double b, const double *y,
+                            hipStream_t stream) {
+  if (!out || !x) return NRPY_ERR_NULL;
+  if (n == 0) return NRPY_OK;
+  const int B = 256;
+  size_t nb = (n + B - 1) / B;
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL(nrpy_lincomb_kernel, dim3((unsigned)nb), dim3(B), 0, stream, n, out, a, x, b, y);
   return hipGetLastError() == hipSuccess ? NRPY_OK : NRPY_ERR_LAUNCH;
 }
 

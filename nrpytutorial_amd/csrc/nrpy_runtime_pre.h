@@ -42,6 +42,15 @@
 
 #define NRPY_HIP_OK(call) ((call) == hipSuccess)
 
+// Pass-2 helper of the generated vector-Jacobian products: slot k of the pointwise
+// adjoint at the point offset (o0,o1,o2) from (i0,i1,i2), or 0 outside the interior
+// (only interior points carry right-hand sides).
+#define VJP_G_AT(k, o0, o1, o2)                                                                \
+  (((i0) + (o0) >= NGHOSTS && (i0) + (o0) < Nxx0 + NGHOSTS && (i1) + (o1) >= NGHOSTS &&       \
+    (i1) + (o1) < Nxx1 + NGHOSTS && (i2) + (o2) >= NGHOSTS && (i2) + (o2) < Nxx2 + NGHOSTS)    \
+       ? vjp_G[IDX4(k, (i0) + (o0), (i1) + (o1), (i2) + (o2))]                               \
+       : 0.0)
+
 // Launch geometry. Kernels run on a 1-D grid of blocks of shape
 // (BX, BY, BZ) tiling the (n0, n1, n2) index box. The hardware hands block b
 // to XCD b % 8, so consecutive blocks land on different L2 caches; the remap

@@ -179,13 +179,13 @@ def _read_statement(gfname, idx, array):
     return "const double %s%s = %s[IDX4(%sGF, %s)];" % (gfname, _suffix(idx), array, gfname.upper(), _ijk(idx))
 
 
-def FD_outputC(filename, sympyexpr_list, params="", upwindcontrolvec=""):
-    """Generate the per-gridpoint block for a list of ``lhrh`` pairs."""
-    p = parse_outCparams_string(params)
+def FD_outputC_readblock(sympyexpr_list, upwindcontrolvec=""):
+    """Steps 1-2 of FD_outputC for ``sympyexpr_list``: memory reads, finite-difference
+    stencils (CSE prefix FDPart1) and upwinding (FDPart2). Returns (C code, number of
+    steps emitted, derivative symbols). The adjoint generator reuses it to rebuild the
+    forward quantities of a right-hand side before its reverse sweep."""
     if not isinstance(sympyexpr_list, list):
         sympyexpr_list = [sympyexpr_list]
-    indent = "  " if p.includebraces == "True" else ""
-    fullindent = indent + p.preindent
     derivs = list_deriv_vars(sympyexpr_list, upwindcontrolvec)
     bases, ops = [], []
     for v in derivs:
@@ -206,7 +206,6 @@ def FD_outputC(filename, sympyexpr_list, params="", upwindcontrolvec=""):
         for v in sp.sympify(e.rhs).free_symbols:
             if gri.variable_type(v) == "gridfunction":
                 points.setdefault(str(v), []).append((0, 0, 0, 0))
-    evol, aux, auxevol = gri.gridfunction_lists()
     order = {n: k for k, n in enumerate(gri.glb_gridfcs_list and [g.name for g in gri.glb_gridfcs_list])}
     reads = []
     for gf in sorted(points, key=lambda n: order.get(n, 0)):
@@ -265,6 +264,17 @@ def FD_outputC(filename, sympyexpr_list, params="", upwindcontrolvec=""):
         if up_exprs:
             out += outputC(up_exprs, up_names, "returnstring",
                            params="includebraces=False,CSE_varprefix=FDPart2,outCverbose=False")
+    return out, step, nsteps, derivs
+
+
+def FD_outputC(filename, sympyexpr_list, params="", upwindcontrolvec=""):
+    """Generate the per-gridpoint block for a list of ``lhrh`` pairs."""
+    p = parse_outCparams_string(params)
+    if not isinstance(sympyexpr_list, list):
+        sympyexpr_list = [sympyexpr_list]
+    indent = "  " if p.includebraces == "True" else ""
+    fullindent = indent + p.preindent
+    out, step, nsteps, _ = FD_outputC_readblock(sympyexpr_list, upwindcontrolvec)
     out += "/* Step %d of %d: evaluate right-hand sides and write to memory */\n" % (step, nsteps)
     out += outputC([e.rhs for e in sympyexpr_list], [e.lhs for e in sympyexpr_list], "returnstring",
                    params="includebraces=False,CSE_varprefix=FDPart3,CSE_preprocess=True,outCverbose=False")

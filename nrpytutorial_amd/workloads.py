@@ -10,6 +10,9 @@ reproducible across numpy/torch versions.
   to the flat BSSN state (h=a=lambda=vet=bet=trK=0, cf=alpha=1): the
   "flat metric plus a sum of 8 seeded Gaussians in every field" input of the
   192^3 BSSN RHS benchmark.
+* ``scalarwave_bump_parameters`` / ``scalarwave_bump_basis_*``: the adjoint
+  scalar-wave input uu = 2 + sum_k a_k exp(-|x - c_k|^2 / (2 * 1.5^2)), vv = 0,
+  16 bumps with centres in [-5,5]^3.
 """
 import math
 
@@ -86,3 +89,34 @@ def bssn_gaussian_state_torch(out, xx_dev, gf_names, seed):
         for A, w, cx, cy, cz in bumps[g]:
             acc.add_(A * ((-((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2) / (w * w)).exp()))
     return out
+
+
+def scalarwave_bump_parameters(seed, nbumps=16, cmin=-5.0, cmax=5.0, amin=0.5, amax=1.5):
+    """Adjoint scalar-wave input: bump centres uniform in [cmin,cmax]^3 and amplitudes
+    uniform in [amin,amax] (the amplitude range is this build's choice: the workload fixes
+    only the centres and the width). Returns (amplitudes [n], centres [n][3])."""
+    u = uniforms(seed, nbumps * 4)
+    amps = [amin + (amax - amin) * u[4 * k] for k in range(nbumps)]
+    cents = [[cmin + (cmax - cmin) * u[4 * k + 1 + d] for d in range(3)] for k in range(nbumps)]
+    return amps, cents
+
+
+def scalarwave_bump_basis_torch(xx_dev, centres, width=1.5):
+    """(nbumps, N2, N1, N0) device tensor of exp(-|x - c_k|^2 / (2 width^2))."""
+    import torch
+    X = xx_dev[0][None, None, :]
+    Y = xx_dev[1][None, :, None]
+    Z = xx_dev[2][:, None, None]
+    out = torch.empty((len(centres), len(xx_dev[2]), len(xx_dev[1]), len(xx_dev[0])), dtype=torch.float64,
+                      device=xx_dev[0].device)
+    for k, (cx, cy, cz) in enumerate(centres):
+        out[k] = (-((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2) / (2.0 * width * width)).exp()
+    return out
+
+
+def scalarwave_bump_basis_numpy(xx, centres, width=1.5):
+    X = xx[0][None, None, :]
+    Y = xx[1][None, :, None]
+    Z = xx[2][:, None, None]
+    return np.stack([np.exp(-((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2) / (2.0 * width * width))
+                     for cx, cy, cz in centres])
