@@ -160,6 +160,10 @@ int bc_apply_T(const paramstruct *params, const bc_struct *bc, const int NUM_GFS
 int nrpy_lincomb(size_t n, double *out, double a, const double *x, double b, const double *y,
                  hipStream_t stream);
 int nrpy_fill(size_t n, double *out, double v, hipStream_t stream);
+/* acc = (first ? L : acc) + w and, if kbar is not NULL, kbar = a*L + b*w: one pass of the
+ * RK4 step adjoint's stage bookkeeping. */
+int nrpy_adjoint_combine(size_t n, double *acc, const double *L, const double *w, double *kbar, double a, double b,
+                         int first, hipStream_t stream);
 
 /* ---- diagnostics and tuning ---------------------------------------------- */
 

@@ -58,6 +58,27 @@ int Hamiltonian_constraint(const rfm_struct *rfmstruct, const paramstruct *param
 int enforce_detgammabar_constraint(const rfm_struct *rfmstruct, const paramstruct *params, double *in_gfs,
                                    hipStream_t stream);
 
+/* Batched evolutions (no counterpart in the reference): ninst independent grids of this
+ * system (own sizes, parameters, boundary maps, rfm tables, dt), one launch per kernel
+ * per RK4 substep for all of them. params: host array of ninst paramstructs; rfms, bcs:
+ * host arrays of the per-instance handles; arrays: host array of 6*ninst device
+ * pointers, instance k's y_n, k_odd, k_even, y_nplus1_running_total, auxevol, aux;
+ * dts: host array of time steps. The batch keeps copies of params and handles: the
+ * per-instance rfm/bc handles and arrays must outlive it. */
+typedef struct nrpy_batch nrpy_batch;
+int nrpy_batch_create(const int ninst, const paramstruct *params, const rfm_struct *const *rfms,
+                      const bc_struct *const *bcs, double *const *arrays, const double *dts, nrpy_batch **out);
+void nrpy_batch_free(nrpy_batch *b);
+/* Rebuild the tile tables after nrpy_set_block_shape on a batched kernel id. */
+int nrpy_batch_prepare(nrpy_batch *b);
+/* Composed ghost fill of array role (0..5) of every instance. */
+int nrpy_batch_apply_bcs(const nrpy_batch *b, const int role, const int NUM_GFS, const int8_t *gfs_parity,
+                         hipStream_t stream);
+/* One RK4 step of every instance (as MoL_step_forward_in_time), and its hipGraph
+ * (replay with MoL_graph_launch). */
+int MoL_batch_step(nrpy_batch *b, hipStream_t stream);
+int MoL_batch_graph_create(nrpy_batch *b, MoL_graph **out);
+
 /* Replaces initial_data(const paramstruct*, REAL *xx[3], REAL *in_gfs)
  * (BSSN_ID_function_string.py:9-39) for Brill-Lindquist data
  * (BrillLindquist.py:36-82); xx are device coordinate arrays. */

@@ -267,6 +267,13 @@ def fused_rk4_kernel(name, args_decl, call_args, body_with_writes, loopopts="Int
     if "rhs_gfs" in fused_body:
         raise ValueError("unrecognised write to rhs_gfs in the RHS body")
     kbody = lp.simple_loop(loopopts, fused_body)
+    arrays = [("const double *" if "const" in a else "double *", a.split()[-1])
+              for a in args_decl.split(",") if "*" in a]
+    # rk_yn and rk_out are the same array in the last substep: no __restrict__ on the RK arrays.
+    lp.BATCH_SPECS[name] = {"arrays": [(q + "__restrict__ ", an) for q, an in arrays] +
+                                      [("double *", "rk_yn"), ("double *", "rk_rt"), ("double *", "rk_out")],
+                            "rfm": "rfm_struct" in args_decl, "template": "S", "body": fused_body,
+                            "loopopts": loopopts, "dt": True}
     unpack = "".join("  const %s %s = params.%s;\n" % (lp._ctype(t), n, n) for n, t in lp._used_cparams(kbody))
     for n in ("Nxx0", "Nxx1", "Nxx2", "Nxx_plus_2NGHOSTS0", "Nxx_plus_2NGHOSTS1", "Nxx_plus_2NGHOSTS2"):
         if not re.search(r"const int %s = " % n, unpack):

@@ -32,6 +32,7 @@ from .. import outputC as oc
 from ..outputC import lhrh
 from . import bundle as bd
 from . import adjoint as adj
+from . import batch as bt
 
 CURVI_SCALARWAVE = {"scalarwave_sinhsphericalv2": "SinhSphericalv2",
                     "scalarwave_sinhcylindricalv2": "SinhCylindricalv2"}
@@ -98,7 +99,52 @@ def _mol_glue(has_ricci, has_detg, rhs_args_decl, rhs_call_args):
         "  if (!g) return;\n  (void)hipGraphExecDestroy(g->exec);\n  (void)hipGraphDestroy(g->graph);\n  delete g;\n}\n")
 
 
-def _mol_adjoint_glue(has_rfm_vjp):
+def _mol_batch_glue(n_ricci, n_rhs):
+    """MoL_batch_step / MoL_batch_graph_create: one RK4 step of every instance of a batch,
+    the diagonal schedule with one launch per kernel per substep for all instances."""
+    sub = ""
+    for s, (i, o) in enumerate((("NRPY_BATCH_ROLE_YN", "NRPY_BATCH_ROLE_KODD"),
+                                ("NRPY_BATCH_ROLE_KODD", "NRPY_BATCH_ROLE_KEVEN"),
+                                ("NRPY_BATCH_ROLE_KEVEN", "NRPY_BATCH_ROLE_KODD"),
+                                ("NRPY_BATCH_ROLE_KODD", "NRPY_BATCH_ROLE_YN"))):
+        sub += "  /* substep %d */\n" % s
+        for k in range(n_ricci):
+            sub += ("  if ((rc = Ricci_eval_g%d_batched(b, %s, NRPY_BATCH_ROLE_AUXEVOL, stream))) return rc;\n"
+                    % (k, i))
+        for k in range(n_rhs):
+            sub += ("  if ((rc = rhs_eval_rk4_substep_g%d_batched(%d, b, NRPY_BATCH_ROLE_AUXEVOL, %s, NRPY_BATCH_ROLE_YN, "
+                    "NRPY_BATCH_ROLE_RT, %s, stream))) return rc;\n" % (k, s, i, o))
+        sub += ("  if ((rc = nrpy_batch_apply_bcs(b, %s, NUM_EVOL_GFS, evol_gf_parity, stream))) return rc;\n"
+                "  if ((rc = enforce_detgammabar_constraint_batched(b, %s, stream))) return rc;\n" % (o, o))
+    return (
+        "// One RK4 step of every instance of the batch (their own dt), as MoL_step_forward_in_time does\n"
+        "// for one grid.\n"
+        'extern "C" int MoL_batch_step(nrpy_batch *b, hipStream_t stream) {\n'
+        "  if (!b) return NRPY_ERR_NULL;\n  int rc;\n" + sub + "  return NRPY_OK;\n}\n"
+        "// The batch step captured into a hipGraph; replay with MoL_graph_launch, free with MoL_graph_free.\n"
+        'extern "C" int MoL_batch_graph_create(nrpy_batch *b, MoL_graph **out) {\n'
+        "  if (!b || !out) return NRPY_ERR_NULL;\n  *out = nullptr;\n"
+        "  int rc = nrpy_batch_prepare(b);\n  if (rc) return rc;\n"
+        "  hipStream_t s;\n"
+        "  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return NRPY_ERR_LAUNCH;\n"
+        "  MoL_graph *g = new MoL_graph();\n"
+        "  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = NRPY_ERR_LAUNCH;\n"
+        "  if (!rc) {\n"
+        "    rc = MoL_batch_step(b, s);\n"
+        "    hipGraph_t gr;\n"
+        "    const hipError_t e = hipStreamEndCapture(s, &gr);\n"
+        "    if (!rc && e != hipSuccess) rc = NRPY_ERR_LAUNCH;\n"
+        "    if (!rc) {\n"
+        "      g->graph = gr;\n"
+        "      if (hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0) != hipSuccess) rc = NRPY_ERR_LAUNCH;\n"
+        "    }\n"
+        "  }\n"
+        "  (void)hipStreamDestroy(s);\n"
+        "  if (rc) { delete g; return rc; }\n"
+        "  *out = g;\n  return NRPY_OK;\n}\n")
+
+
+def _mol_adjoint_glue(has_rfm_vjp, state_dependent, writes_all):
     """MoL_step_adjoint: the adjoint of one RK4 step (diagonal schedule, ghost fill after
     every substep) for systems whose step is [RHS + update -> ghost fill] per substep.
 
@@ -108,12 +154,34 @@ def _mol_adjoint_glue(has_rfm_vjp):
     so for the cotangent L of u_new, with w_s = E^T R'(E u_s)^T kbar_s:
       kbar4 = dt/6 L, kbar3 = dt/3 L + dt w3, kbar2 = dt/3 L + dt/2 w2, kbar1 = dt/6 L + dt/2 w1,
       L_n = L + w3 + w2 + w1 + w0.
-    The stage states are recomputed from y_n with the forward substep kernels."""
+    When the right-hand side depends on the state (state_dependent), the stage states are
+    recomputed from y_n with the forward substep kernels; for a linear right-hand side the
+    adjoint does not read them. When the VJP writes every evolved gridfunction
+    (writes_all), its output needs no clearing."""
     rfm_arg = "rfmstruct, " if has_rfm_vjp else ""
-    vjp = ("  if ((rc = nrpy_fill(n, w, 0.0, stream))) return rc;\n"
+    vjp = (("" if writes_all else "  if ((rc = nrpy_fill(n, w, 0.0, stream))) return rc;\n") +
            "  if ((rc = rhs_eval_vjp(" + rfm_arg + "params, auxevol_gfs, STATE, kbar, vjp_G, w, nullptr, stream))) "
            "return rc;\n"
            "  if ((rc = bc_apply_T(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, w, stream))) return rc;\n")
+    if state_dependent:
+        recompute = (
+            "  double *yn = (double *)y_n_gfs;  // substeps 0-2 read y_n only\n"
+            "  if ((rc = rhs_eval_rk4_substep(0, rfmstruct, params, auxevol_gfs, yn, yn, rt, U1, dt, stream))) "
+            "return rc;\n"
+            "  if ((rc = apply_bcs_curvilinear_composed(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, U1, stream))) "
+            "return rc;\n"
+            "  if ((rc = rhs_eval_rk4_substep(1, rfmstruct, params, auxevol_gfs, U1, yn, rt, U2, dt, stream))) "
+            "return rc;\n"
+            "  if ((rc = apply_bcs_curvilinear_composed(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, U2, stream))) "
+            "return rc;\n"
+            "  if ((rc = rhs_eval_rk4_substep(2, rfmstruct, params, auxevol_gfs, U2, yn, rt, U3, dt, stream))) "
+            "return rc;\n"
+            "  if ((rc = apply_bcs_curvilinear_composed(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, U3, stream))) "
+            "return rc;\n")
+        states = ("U3", "U2", "U1", "y_n_gfs")
+    else:
+        recompute = "  (void)U1; (void)U2; (void)U3; (void)rt;  // linear right-hand side: the adjoint ignores the state\n"
+        states = ("y_n_gfs",) * 4
     return (
         "// Adjoint of one RK4 step. On entry lambda_gfs holds the cotangent of y_{n+1} (interior points;\n"
         "// ghost cotangents zero), y_n_gfs the state at the start of the step with its ghosts filled. On\n"
@@ -128,31 +196,18 @@ def _mol_adjoint_glue(has_rfm_vjp):
         "  int rc;\n"
         "  const size_t n = (size_t)NUM_EVOL_GFS * params->Nxx_plus_2NGHOSTS0 * params->Nxx_plus_2NGHOSTS1 *\n"
         "                   params->Nxx_plus_2NGHOSTS2;\n"
-        "  double *yn = (double *)y_n_gfs;  // substeps 0-2 read y_n only\n"
-        "  if ((rc = rhs_eval_rk4_substep(0, rfmstruct, params, auxevol_gfs, yn, yn, rt, U1, dt, stream))) return rc;\n"
-        "  if ((rc = apply_bcs_curvilinear_composed(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, U1, stream))) "
-        "return rc;\n"
-        "  if ((rc = rhs_eval_rk4_substep(1, rfmstruct, params, auxevol_gfs, U1, yn, rt, U2, dt, stream))) return rc;\n"
-        "  if ((rc = apply_bcs_curvilinear_composed(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, U2, stream))) "
-        "return rc;\n"
-        "  if ((rc = rhs_eval_rk4_substep(2, rfmstruct, params, auxevol_gfs, U2, yn, rt, U3, dt, stream))) return rc;\n"
-        "  if ((rc = apply_bcs_curvilinear_composed(params, bcstruct, NUM_EVOL_GFS, evol_gf_parity, U3, stream))) "
-        "return rc;\n"
-        "  /* stage 4 */\n"
+        + recompute +
+        "  /* stage 4: kbar4 = dt/6 L */\n"
         "  if ((rc = nrpy_lincomb(n, kbar, dt / 6.0, lambda_gfs, 0.0, nullptr, stream))) return rc;\n"
-        + vjp.replace("STATE", "U3") +
-        "  if ((rc = nrpy_lincomb(n, acc, 1.0, lambda_gfs, 1.0, w, stream))) return rc;\n"
-        "  /* stage 3 */\n"
-        "  if ((rc = nrpy_lincomb(n, kbar, dt / 3.0, lambda_gfs, dt, w, stream))) return rc;\n"
-        + vjp.replace("STATE", "U2") +
-        "  if ((rc = nrpy_lincomb(n, acc, 1.0, acc, 1.0, w, stream))) return rc;\n"
-        "  /* stage 2 */\n"
-        "  if ((rc = nrpy_lincomb(n, kbar, dt / 3.0, lambda_gfs, dt / 2.0, w, stream))) return rc;\n"
-        + vjp.replace("STATE", "U1") +
-        "  if ((rc = nrpy_lincomb(n, acc, 1.0, acc, 1.0, w, stream))) return rc;\n"
-        "  /* stage 1 */\n"
-        "  if ((rc = nrpy_lincomb(n, kbar, dt / 6.0, lambda_gfs, dt / 2.0, w, stream))) return rc;\n"
-        + vjp.replace("STATE", "y_n_gfs") +
+        + vjp.replace("STATE", states[0]) +
+        "  /* acc = L + w3; kbar3 = dt/3 L + dt w3 */\n"
+        "  if ((rc = nrpy_adjoint_combine(n, acc, lambda_gfs, w, kbar, dt / 3.0, dt, 1, stream))) return rc;\n"
+        + vjp.replace("STATE", states[1]) +
+        "  if ((rc = nrpy_adjoint_combine(n, acc, lambda_gfs, w, kbar, dt / 3.0, dt / 2.0, 0, stream))) return rc;\n"
+        + vjp.replace("STATE", states[2]) +
+        "  if ((rc = nrpy_adjoint_combine(n, acc, lambda_gfs, w, kbar, dt / 6.0, dt / 2.0, 0, stream))) return rc;\n"
+        + vjp.replace("STATE", states[3]) +
+        "  /* L_n = acc + w0 */\n"
         "  return nrpy_lincomb(n, lambda_gfs, 1.0, acc, 1.0, w, stream);\n"
         "}\n")
 
@@ -264,11 +319,12 @@ def scalarwave_cartesian():
     fused += _fused_launcher("rhs_eval_rk4_substep", "const rfm_struct *rfmstruct, const paramstruct *params, "
                              "const double *auxevol_gfs, const double *in_gfs", "*params, in_gfs", "InteriorPoints")
     id_src = _initial_data_kernel([swid.uu_ID, swid.vv_ID], ["uu", "vv"])
-    vjp_src, slots, _ = adj.vjp_kernels("rhs_eval", exprs, "", "InteriorPoints", "")
+    vjp_src, slots, vouts, vdep = adj.vjp_kernels("rhs_eval", exprs, "", "InteriorPoints", "")
     pre, meta = _header(3)
     parts = [pre, bd.coord_host_functions("Cartesian"), '#include "../nrpy_runtime.h"', bd.rfm_struct_code(),
              rhs_src, fused, id_src, vjp_src,
-             _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN"), _mol_adjoint_glue(False)]
+             _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN"),
+             _mol_adjoint_glue(False, vdep, sorted(vouts["EVOL"]) == sorted(gri.gridfunction_lists()[0]))]
     meta["vjp_slots"] = [sl[0] for sl in slots]
     meta["cost"] = {"rhs_eval": bd.count_body(body)}
     meta.update({"bundle": "scalarwave_cartesian", "CoordSystem": "Cartesian", "rfm_tables": [],
@@ -313,9 +369,10 @@ def maxwell_cartesian():
     id_src = _initial_data_kernel(id_rhss, names)
     coords = bd.coord_host_functions("Cartesian")
     pre, meta = _header(3)
-    vjp_src, slots, _ = adj.vjp_kernels("rhs_eval", exprs, "", "InteriorPoints", "")
+    vjp_src, slots, vouts, vdep = adj.vjp_kernels("rhs_eval", exprs, "", "InteriorPoints", "")
     parts = [pre, coords, '#include "../nrpy_runtime.h"', bd.rfm_struct_code(), rhs_src, con_src, fused, id_src,
-             vjp_src, _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN"), _mol_adjoint_glue(False)]
+             vjp_src, _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN"),
+             _mol_adjoint_glue(False, vdep, sorted(vouts["EVOL"]) == sorted(gri.gridfunction_lists()[0]))]
     meta["vjp_slots"] = [sl[0] for sl in slots]
     meta["cost"] = {"rhs_eval": bd.count_body(body)}
     meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
@@ -493,10 +550,15 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     t0 = time.time()
     id_src = _initial_data_kernel(id_rhss, id_names)
     print("(BENCH) initial data codegen %.1fs" % (time.time() - t0), flush=True)
+    batch_names = (["Ricci_eval_g%d" % k for k in range(len(ric_groups))] +
+                   ["rhs_eval_rk4_substep_g%d" % k for k in range(len(groups))] + ["enforce_detgammabar_constraint"])
+    batch_src, batch_table = bt.batched_kernels(batch_names)
     pre, meta = _header(FD_order // 2 + 1)
     parts = [pre, bd.coord_host_functions(CoordSystem), '#include "../nrpy_runtime.h"', rfm_code,
              ricci_src, rhs_src, fused, h_src, detg_src, id_src,
-             _mol_glue(True, True, None, "rfmstruct, params, auxevol_gfs, IN")]
+             _mol_glue(True, True, None, "rfmstruct, params, auxevol_gfs, IN"),
+             bt.instance_struct(), batch_table, '#include "../nrpy_batch.h"', batch_src,
+             _mol_batch_glue(len(ric_groups), len(groups))]
     meta["rhs_groups"] = groups
     meta["ricci_groups"] = ric_groups
     meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)

@@ -650,6 +650,30 @@ extern "C" int nrpy_fill(size_t n, double *out, double v, hipStream_t stream) {
   return hipGetLastError() == hipSuccess ? NRPY_OK : NRPY_ERR_LAUNCH;
 }
 
+__global__ void nrpy_adjoint_combine_kernel(size_t n, double *__restrict__ acc, const double *__restrict__ L,
+                                            const double *__restrict__ w, double *__restrict__ kbar, double a,
+                                            double b, int first) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double wi = w[i], Li = L[i];
+    acc[i] = (first ? Li : acc[i]) + wi;
+    if (kbar) kbar[i] = a * Li + b * wi;
+  }
+}
+
+// One pass of the RK4 step adjoint: acc = (first ? L : acc) + w and, if kbar is not
+// NULL, kbar = a*L + b*w (the next stage's RHS cotangent).
+extern "C" int nrpy_adjoint_combine(size_t n, double *acc, const double *L, const double *w, double *kbar, double a,
+                                    double b, int first, hipStream_t stream) {
+  if (!acc || !L || !w) return NRPY_ERR_NULL;
+  if (n == 0) return NRPY_OK;
+  const int B = 256;
+  size_t nb = (n + B - 1) / B;
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL(nrpy_adjoint_combine_kernel, dim3((unsigned)nb), dim3(B), 0, stream, n, acc, L, w, kbar, a, b,
+                     first);
+  return hipGetLastError() == hipSuccess ? NRPY_OK : NRPY_ERR_LAUNCH;
+}
+
 // out[i] = a*x[i] + b*y[i] for i < n (y may be NULL: out = a*x). out may alias x or y.
 extern "C" int nrpy_lincomb(size_t n, double *out, double a, const double *x, double b, const double *y,
                             hipStream_t stream) {

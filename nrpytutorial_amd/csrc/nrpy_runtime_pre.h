@@ -72,6 +72,16 @@
     nrpy_bz = t_ / (nbx_ * nby_);                                                              \
   }
 
+// Batched launches (nrpy_batch.h): array roles of an instance and index ranges.
+#define NRPY_BATCH_ROLE_YN 0
+#define NRPY_BATCH_ROLE_KODD 1
+#define NRPY_BATCH_ROLE_KEVEN 2
+#define NRPY_BATCH_ROLE_RT 3
+#define NRPY_BATCH_ROLE_AUXEVOL 4
+#define NRPY_BATCH_ROLE_AUX 5
+#define NRPY_RANGE_INTERIOR 0
+#define NRPY_RANGE_ALL 1
+
 // Per-kernel block shapes, settable at run time (nrpy_set_block_shape) so an
 // autotuner can pick them; index by NRPY_KERNEL_* ids defined per bundle.
 #define NRPY_MAX_KERNELS 64

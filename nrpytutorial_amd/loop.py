@@ -35,6 +35,8 @@ def launch_bounds():
     return "%d, %d" % (MAX_THREADS, int(w)) if w else "%d" % MAX_THREADS
 # kernel name -> id into the runtime's tunable block-shape table
 KERNEL_IDS = {}
+# kernel name -> what is needed to emit its batched (many-instance) variant
+BATCH_SPECS = {}
 
 
 def kernel_id(name):
@@ -102,15 +104,21 @@ def rfm_reads(body):
     return out
 
 
-def simple_loop(options, interior):
-    """Kernel body: lane -> gridpoint mapping, bounds check, coordinate/table reads, ``interior``."""
+def simple_loop(options, interior, batched=False):
+    """Kernel body: lane -> gridpoint mapping, bounds check, coordinate/table reads, ``interior``.
+
+    batched=True: the block's tile comes from the instance table of a batched launch
+    (NRPY_BATCH_TILE, defined by the batched kernel prologue) instead of the XCD remap."""
     o = parse_loopopts(options)
     lo = "NGHOSTS" if o["range"] == "interior" else "0"
     n = ("Nxx%d" if o["range"] == "interior" else "Nxx_plus_2NGHOSTS%d")
     # One-dimensional grid of blocks; NRPY_BLOCK_COORDS remaps the hardware's
     # round-robin block-to-XCD assignment to contiguous tiles per XCD and
     # decodes the (bx, by, bz) tile of this block.
-    s = "NRPY_BLOCK_COORDS(%s, %s, %s);\n" % (n % 0, n % 1, n % 2)
+    if batched:
+        s = "NRPY_BATCH_TILE(%s, %s, %s);\n" % (n % 0, n % 1, n % 2)
+    else:
+        s = "NRPY_BLOCK_COORDS(%s, %s, %s);\n" % (n % 0, n % 1, n % 2)
     for d, t in ((0, "x"), (1, "y"), (2, "z")):
         s += "const int i%d = %s + nrpy_b%s*(int)blockDim.%s + (int)threadIdx.%s;\n" % (d, lo, t, t, t)
     if o["range"] == "interior":
@@ -152,6 +160,21 @@ def kernel_and_launcher(name, desc, args, preloop, body, loopopts, postloop):
     o = parse_loopopts(loopopts)
     kbody = simple_loop(loopopts, body)
     full = preloop + "\n" + kbody + "\n" + postloop
+    arrays, has_rfm, batchable = [], False, True
+    for ctype, an in args:
+        base = ctype.replace("const", "").replace("*", "").strip()
+        if base == "paramstruct":
+            continue
+        if base == "rfm_struct":
+            has_rfm = True
+        elif an == "xx" and "[3]" in ctype:
+            batchable = False
+        else:
+            arrays.append(("const double *__restrict__ " if "const" in ctype else "double *__restrict__ ", an))
+    if batchable:
+        BATCH_SPECS[name] = {"arrays": arrays, "rfm": has_rfm, "template": None,
+                             "body": preloop + "\n" + body + "\n" + postloop if (preloop or postloop) else body,
+                             "loopopts": loopopts, "dt": False}
     kargs, largs, call = [], [], []
     for ctype, an in args:
         base = ctype.replace("const", "").replace("*", "").strip()
