@@ -58,6 +58,19 @@ int Hamiltonian_constraint(const rfm_struct *rfmstruct, const paramstruct *param
 int enforce_detgammabar_constraint(const rfm_struct *rfmstruct, const paramstruct *params, double *in_gfs,
                                    hipStream_t stream);
 
+/* Vector-Jacobian products (no counterpart in the reference; see rhs_eval_vjp in
+ * nrpy_amd_scalarwave.h for the conventions). rhs_eval_vjp: cotangent of the 24 RHS
+ * outputs -> cotangents of in_gfs (cot_in_gfs) and of RbarDD (cot_auxevol_gfs);
+ * vjp_G holds <bundle>.json "vjp_slots" gridfunctions. Ricci_eval_vjp: cotangent of
+ * the six RbarDD -> cotangent of in_gfs; vjp_G holds "ricci_vjp_slots" gridfunctions,
+ * auxevol_gfs and cot_auxevol_gfs are unused (may be NULL). */
+int rhs_eval_vjp(const rfm_struct *rfmstruct, const paramstruct *params, const double *auxevol_gfs,
+                 const double *in_gfs, const double *cot_rhs_gfs, double *vjp_G, double *cot_in_gfs,
+                 double *cot_auxevol_gfs, hipStream_t stream);
+int Ricci_eval_vjp(const rfm_struct *rfmstruct, const paramstruct *params, const double *auxevol_gfs,
+                   const double *in_gfs, const double *cot_rhs_gfs, double *vjp_G, double *cot_in_gfs,
+                   double *cot_auxevol_gfs, hipStream_t stream);
+
 /* Batched evolutions (no counterpart in the reference): ninst independent grids of this
  * system (own sizes, parameters, boundary maps, rfm tables, dt), one launch per kernel
  * per RK4 substep for all of them. params: host array of ninst paramstructs; rfms, bcs:

@@ -550,17 +550,28 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     t0 = time.time()
     id_src = _initial_data_kernel(id_rhss, id_names)
     print("(BENCH) initial data codegen %.1fs" % (time.time() - t0), flush=True)
+    # Vector-Jacobian products of the whole RHS (all outputs, inputs in_gfs and RbarDD) and of
+    # the Ricci tensor (inputs in_gfs), for adjoints through rhs_eval and Ricci_eval.
+    t0 = time.time()
+    vjp_src, vslots, _, _ = adj.vjp_kernels("rhs_eval", evol_rhss, betaU, "InteriorPoints,Enable_rfm_precompute",
+                                            "rfm_struct *restrict rfmstruct,")
+    rvjp_src, rslots, _, _ = adj.vjp_kernels(
+        "Ricci_eval", [lhrh(lhs=gri.gfaccess("auxevol_gfs", n), rhs=ric_exprs[n]) for n in ric_names], "",
+        "InteriorPoints,Enable_rfm_precompute", "rfm_struct *restrict rfmstruct,")
+    print("(BENCH) VJP codegen %.1fs (%d + %d slots)" % (time.time() - t0, len(vslots), len(rslots)), flush=True)
     batch_names = (["Ricci_eval_g%d" % k for k in range(len(ric_groups))] +
                    ["rhs_eval_rk4_substep_g%d" % k for k in range(len(groups))] + ["enforce_detgammabar_constraint"])
     batch_src, batch_table = bt.batched_kernels(batch_names)
     pre, meta = _header(FD_order // 2 + 1)
     parts = [pre, bd.coord_host_functions(CoordSystem), '#include "../nrpy_runtime.h"', rfm_code,
-             ricci_src, rhs_src, fused, h_src, detg_src, id_src,
+             ricci_src, rhs_src, fused, h_src, detg_src, id_src, vjp_src, rvjp_src,
              _mol_glue(True, True, None, "rfmstruct, params, auxevol_gfs, IN"),
              bt.instance_struct(), batch_table, '#include "../nrpy_batch.h"', batch_src,
              _mol_batch_glue(len(ric_groups), len(groups))]
     meta["rhs_groups"] = groups
     meta["ricci_groups"] = ric_groups
+    meta["vjp_slots"] = [sl[0] for sl in vslots]
+    meta["ricci_vjp_slots"] = [sl[0] for sl in rslots]
     meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
     meta["cost"] = {"rhs_eval": {"flops": sum(c["flops"] for c in group_costs), "groups": group_costs},
                     "Ricci_eval": {"flops": sum(c["flops"] for c in ricci_costs), "groups": ricci_costs},

@@ -78,6 +78,7 @@ KERNEL_PROTOTYPES = {
     "scalarwave": {"rhs_eval": [_VP, _VP, _VP, _VP]},
     "scalarwave_curvilinear": {"rhs_eval": [_VP, _VP, _VP, _VP, _VP]},
     "bssn": {"rhs_eval": [_VP, _VP, _VP, _VP, _VP, _VP], "Ricci_eval": [_VP, _VP, _VP, _VP, _VP],
+             "rhs_eval_vjp": [_VP] * 9, "Ricci_eval_vjp": [_VP] * 9,
              "Hamiltonian_constraint": [_VP, _VP, _VP, _VP, _VP],
              "enforce_detgammabar_constraint": [_VP, _VP, _VP, _VP]},
     "maxwell": {"rhs_eval": [_VP, _VP, _VP, _VP], "Maxwell_constraints": [_VP, _VP, _VP, _VP]},
