@@ -71,6 +71,21 @@ int Ricci_eval_vjp(const rfm_struct *rfmstruct, const paramstruct *params, const
                    const double *in_gfs, const double *cot_rhs_gfs, double *vjp_G, double *cot_in_gfs,
                    double *cot_auxevol_gfs, hipStream_t stream);
 
+/* For the adjoint of the BSSN RK4 step and of the H-L2 diagnostic (no counterparts in the
+ * reference): the VJP of enforce_detgammabar_constraint (pointwise at all points; writes
+ * only the hDD cotangents of cot_in_gfs, vjp_G: "detg_vjp_slots"), the VJP of
+ * Hamiltonian_constraint (cotangent of H -> cotangent of in_gfs, "hamiltonian_vjp_slots"),
+ * and the derivatives of the Brill-Lindquist initial data with respect to BH1_mass and
+ * BH2_mass at all points (device xx arrays). */
+int enforce_detgammabar_constraint_vjp(const rfm_struct *rfmstruct, const paramstruct *params,
+                                       const double *auxevol_gfs, const double *in_gfs, const double *cot_rhs_gfs,
+                                       double *vjp_G, double *cot_in_gfs, double *cot_auxevol_gfs, hipStream_t stream);
+int Hamiltonian_constraint_vjp(const rfm_struct *rfmstruct, const paramstruct *params, const double *auxevol_gfs,
+                               const double *in_gfs, const double *cot_rhs_gfs, double *vjp_G, double *cot_in_gfs,
+                               double *cot_auxevol_gfs, hipStream_t stream);
+int initial_data_dmass(const paramstruct *params, double *xx[3], double *dID_dm1, double *dID_dm2,
+                       hipStream_t stream);
+
 /* Batched evolutions (no counterpart in the reference): ninst independent grids of this
  * system (own sizes, parameters, boundary maps, rfm tables, dt), one launch per kernel
  * per RK4 substep for all of them. params: host array of ninst paramstructs; rfms, bcs:

@@ -131,3 +131,146 @@ def RK4Evolve():
     if _RK4Evolve is None:
         _RK4Evolve = _autograd_function()
     return _RK4Evolve
+
+
+class BSSNStepAdjoint:
+    """Adjoint of the BSSN RK4 step on one Evolution, orchestrated from the host over the
+    bundle's kernels. Each substep of the forward step is
+        X_s = D(E(u_s)),  k_{s+1} = R(X_s, Ric(X_s)),  u_{s+1} = y_n + c_s dt k_{s+1},
+    with E the ghost fill, D the det(gammabar) enforcement (pointwise at every point),
+    Ric the Ricci tensor and R the right-hand side; the step ends with
+    y_{n+1} = D(E(y_n + dt/6 (k1 + 2 k2 + 2 k3 + k4))). The backward pass recomputes the
+    stage states X_s and the pre-enforcement states P_s = E(u_s), then applies the
+    transposes in reverse: D^T (detg VJP at P_s), E^T (bc_apply_T), and
+    R^T + Ric^T (rhs_eval_vjp, whose RbarDD cotangent feeds Ricci_eval_vjp)."""
+
+    def __init__(self, ev):
+        b = ev.g.bundle
+        for key in ("vjp_slots", "ricci_vjp_slots", "detg_vjp_slots"):
+            if key not in b.meta:
+                raise NrpyError("bundle %s has no %s" % (b.name, key))
+        self.ev = ev
+        self.lib = b.lib
+        g = ev.g
+        n = len(b.evol)
+        self.n = n
+        self.X = [g.empty(n) for _ in range(4)]          # X_0 = y_n copy, X_1..X_3
+        self.P = [g.empty(n) for _ in range(5)]          # P_1..P_4 (index 1..4)
+        self.Ric = [g.empty(6) for _ in range(4)]
+        self.rt, self.yw = g.empty(n), g.empty(n)
+        self.acc, self.c, self.cX, self.tmp = (g.empty(n) for _ in range(4))
+        self.ck = [g.empty(n) for _ in range(4)]
+        self.cRic = g.empty(6)
+        nslots = max(len(b.meta["vjp_slots"]), len(b.meta["ricci_vjp_slots"]), len(b.meta["detg_vjp_slots"]),
+                     len(b.meta.get("hamiltonian_vjp_slots", [])))
+        self.G = g.empty(nslots)
+        L = self.lib
+        for f in ("rhs_eval_vjp", "Ricci_eval_vjp", "enforce_detgammabar_constraint_vjp", "Hamiltonian_constraint_vjp"):
+            fn = getattr(L, f)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [_VP] * 9
+        L.bc_apply_T.restype = ctypes.c_int
+        L.bc_apply_T.argtypes = [_VP, _VP, ctypes.c_int, _VP, _VP, _VP]
+        L.rhs_eval_rk4_substep.restype = ctypes.c_int
+
+    def _E(self, y):
+        self.ev.apply_bcs(y, composed=True)
+
+    def _ET(self, c):
+        ev = self.ev
+        check(self.lib.bc_apply_T(ev.g.pp(), ev.g.bc, self.n, ev.g.bundle.evol_parity, ptr(c), ev.s()), "bc_apply_T")
+
+    def _DT(self, P, c):
+        """c <- D'(P)^T c (the enforcement changes only hDD; other cotangents pass through)."""
+        ev = self.ev
+        self.tmp.copy_(c)
+        check(self.lib.enforce_detgammabar_constraint_vjp(ev.g.rfm, ev.g.pp(), None, ptr(P), ptr(c), ptr(self.G),
+                                                          ptr(self.tmp), None, ev.s()), "detg vjp")
+        c.copy_(self.tmp)
+
+    def _RT(self, X, Ric, ck, out):
+        """out <- (R + Ric)'(X)^T ck: full-array cotangent of the stage input X."""
+        ev = self.ev
+        g = ev.g
+        check(self.lib.rhs_eval_vjp(g.rfm, g.pp(), ptr(Ric), ptr(X), ptr(ck), ptr(self.G), ptr(out), ptr(self.cRic),
+                                    ev.s()), "rhs_eval_vjp")
+        self.tmp.zero_()
+        check(self.lib.Ricci_eval_vjp(g.rfm, g.pp(), None, ptr(X), ptr(self.cRic), ptr(self.G), ptr(self.tmp), None,
+                                      ev.s()), "Ricci_eval_vjp")
+        out.add_(self.tmp)
+
+    def recompute(self, y_n, dt):
+        ev, g, L = self.ev, self.ev.g, self.lib
+        self.X[0].copy_(y_n)
+        self.yw.copy_(y_n)
+        for s in range(4):
+            X = self.X[s]
+            check(L.Ricci_eval(g.rfm, g.pp(), ptr(X), ptr(self.Ric[s]), ev.s()), "Ricci_eval")
+            out = self.yw if s == 3 else self.X[s + 1]
+            check(L.rhs_eval_rk4_substep(s, g.rfm, g.pp(), ptr(self.Ric[s]), ptr(X), ptr(self.yw), ptr(self.rt),
+                                         ptr(out), float(dt), ev.s()), "rhs_eval_rk4_substep")
+            self._E(out)
+            self.P[s + 1].copy_(out)
+            ev.post_bcs(out)
+
+    def step_back(self, y_n, lam, dt):
+        """lam (cotangent of y_{n+1}, full array) <- cotangent of y_n (full array)."""
+        self.recompute(y_n, dt)
+        c = self.c
+        c.copy_(lam)
+        self._DT(self.P[4], c)
+        self._ET(c)                                  # cotangent of u4 (interior)
+        self.acc.copy_(c)
+        w = (dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0)
+        for s in range(4):
+            self.ck[s].copy_(c).mul_(w[s])
+        coef = (dt / 2.0, dt / 2.0, dt)               # u_{s+1} = y_n + coef[s] * dt-weighted k_{s+1}
+        for s in (3, 2, 1):                         # stage input X_s -> k_{s+1}
+            self._RT(self.X[s], self.Ric[s], self.ck[s], self.cX)
+            self._DT(self.P[s], self.cX)
+            self._ET(self.cX)                        # cotangent of u_s
+            self.acc.add_(self.cX)
+            self.ck[s - 1].add_(self.cX, alpha=coef[s - 1])
+        self._RT(self.X[0], self.Ric[0], self.ck[0], self.cX)
+        lam.copy_(self.cX).add_(self.acc)
+
+
+def bssn_hl2_mass_gradient(ev, nsteps, CFL=0.5):
+    """J = sqrt(mean_interior H^2) after nsteps RK4 steps of Brill-Lindquist data on ev,
+    and dJ/d(BH1_mass, BH2_mass) by the adjoint (all states of the run are kept: the
+    Brill-Lindquist grids this is meant for are small)."""
+    import torch
+    g, L = ev.g, ev.lib
+    adj = BSSNStepAdjoint(ev)
+    dt = g.find_timestep(CFL)
+    n = len(g.bundle.evol)
+    # initial data, its mass derivatives, y_0 = D(E(ID))
+    ID = g.empty(n)
+    check(L.initial_data(g.pp(), g.xx_dev_ptrs, ptr(ID), ev.s()), "initial_data")
+    dm1, dm2 = g.empty(n), g.empty(n)
+    L.initial_data_dmass.restype = ctypes.c_int
+    check(L.initial_data_dmass(g.pp(), g.xx_dev_ptrs, ptr(dm1), ptr(dm2), ev.s()), "initial_data_dmass")
+    P0 = ID.clone()
+    ev.apply_bcs(P0, composed=True)
+    ev.y_n.copy_(P0)
+    ev.post_bcs(ev.y_n)
+    states = [ev.y_n.clone()]
+    for _ in range(nsteps):
+        ev.step(dt, 1, use_graph=True)
+        states.append(ev.y_n.clone())
+    # loss and its cotangent through the Hamiltonian constraint
+    H = ev.hamiltonian(states[-1])
+    Hi = g.interior(H)
+    J = torch.sqrt(torch.mean(Hi * Hi))
+    cH = torch.zeros_like(ev.aux)
+    g.interior(cH[0]).copy_(Hi / (Hi.numel() * J))
+    lam = torch.zeros_like(ev.y_n)
+    check(L.Hamiltonian_constraint_vjp(g.rfm, g.pp(), None, ptr(states[-1]), ptr(cH), ptr(adj.G), ptr(lam), None,
+                                       ev.s()), "Hamiltonian_constraint_vjp")
+    for s in range(nsteps - 1, -1, -1):
+        adj.step_back(states[s], lam, dt)
+    adj._DT(P0, lam)
+    adj._ET(lam)                                     # cotangent of the interior initial data
+    grad = [float((lam * dm).sum()) for dm in (dm1, dm2)]
+    ev.free_graph()
+    return float(J), grad

@@ -549,6 +549,19 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     rfm.ref_metric__hatted_quantities()
     t0 = time.time()
     id_src = _initial_data_kernel(id_rhss, id_names)
+    # d(initial data)/d(BH1_mass), d/d(BH2_mass) at all points
+    dm_exprs, dm_names = [], []
+    for arr, m in (("dID_dm1", bl.BH1_mass), ("dID_dm2", bl.BH2_mass)):
+        for n, e in sorted(zip(id_names, id_rhss)):
+            dm_exprs.append(sp.diff(e, m))
+            dm_names.append("%s[IDX4(%sGF, i0, i1, i2)]" % (arr, n.upper()))
+    dm_body = oc.outputC(dm_exprs, dm_names, "returnstring", params="includebraces=False,outCverbose=False")
+    id_dmass_src = oc.outCfunction(outfile="returnstring",
+                                   desc="Derivatives of the Brill-Lindquist initial data with respect to the masses.",
+                                   name="initial_data_dmass",
+                                   params="const paramstruct *restrict params,REAL *restrict xx[3], "
+                                          "REAL *restrict dID_dm1, REAL *restrict dID_dm2",
+                                   body=dm_body, loopopts="AllPoints,Read_xxs")
     print("(BENCH) initial data codegen %.1fs" % (time.time() - t0), flush=True)
     # Vector-Jacobian products of the whole RHS (all outputs, inputs in_gfs and RbarDD) and of
     # the Ricci tensor (inputs in_gfs), for adjoints through rhs_eval and Ricci_eval.
@@ -558,13 +571,23 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     rvjp_src, rslots, _, _ = adj.vjp_kernels(
         "Ricci_eval", [lhrh(lhs=gri.gfaccess("auxevol_gfs", n), rhs=ric_exprs[n]) for n in ric_names], "",
         "InteriorPoints,Enable_rfm_precompute", "rfm_struct *restrict rfmstruct,")
-    print("(BENCH) VJP codegen %.1fs (%d + %d slots)" % (time.time() - t0, len(vslots), len(rslots)), flush=True)
+    # det(gammabar) enforcement and the Hamiltonian constraint, for the adjoint of the BSSN RK4
+    # step and of the H-L2 diagnostic; the mass derivatives of the initial data.
+    dvjp_src, dslots, _, _ = adj.vjp_kernels("enforce_detgammabar_constraint", detg_exprs, "",
+                                             "AllPoints,Enable_rfm_precompute", "rfm_struct *restrict rfmstruct,",
+                                             all_points=True)
+    hvjp_src, hslots, _, _ = adj.vjp_kernels("Hamiltonian_constraint", [lhrh(lhs=gri.gfaccess("aux_gfs", "H"),
+                                                                             rhs=bssncon.H)], "",
+                                             "InteriorPoints,Enable_rfm_precompute", "rfm_struct *restrict rfmstruct,")
+    print("(BENCH) VJP codegen %.1fs (%d + %d + %d + %d slots)" % (time.time() - t0, len(vslots), len(rslots),
+                                                                  len(dslots), len(hslots)), flush=True)
     batch_names = (["Ricci_eval_g%d" % k for k in range(len(ric_groups))] +
                    ["rhs_eval_rk4_substep_g%d" % k for k in range(len(groups))] + ["enforce_detgammabar_constraint"])
     batch_src, batch_table = bt.batched_kernels(batch_names)
     pre, meta = _header(FD_order // 2 + 1)
     parts = [pre, bd.coord_host_functions(CoordSystem), '#include "../nrpy_runtime.h"', rfm_code,
-             ricci_src, rhs_src, fused, h_src, detg_src, id_src, vjp_src, rvjp_src,
+             ricci_src, rhs_src, fused, h_src, detg_src, id_src, id_dmass_src, vjp_src, rvjp_src, dvjp_src,
+             hvjp_src,
              _mol_glue(True, True, None, "rfmstruct, params, auxevol_gfs, IN"),
              bt.instance_struct(), batch_table, '#include "../nrpy_batch.h"', batch_src,
              _mol_batch_glue(len(ric_groups), len(groups))]
@@ -572,6 +595,8 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     meta["ricci_groups"] = ric_groups
     meta["vjp_slots"] = [sl[0] for sl in vslots]
     meta["ricci_vjp_slots"] = [sl[0] for sl in rslots]
+    meta["detg_vjp_slots"] = [sl[0] for sl in dslots]
+    meta["hamiltonian_vjp_slots"] = [sl[0] for sl in hslots]
     meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
     meta["cost"] = {"rhs_eval": {"flops": sum(c["flops"] for c in group_costs), "groups": group_costs},
                     "Ricci_eval": {"flops": sum(c["flops"] for c in ricci_costs), "groups": ricci_costs},

@@ -90,3 +90,70 @@ def test_bssn_ricci_vjp_directional(torch_cuda):
     rel = abs(fd - ad) / abs(fd)
     print("directional derivative: FD %.15e  VJP %.15e  rel %.3e" % (fd, ad, rel))
     assert rel < 1e-6
+
+
+BL_NXX = (64, 16, 2)
+
+
+def _bl_masses():
+    from nrpytutorial_amd.workloads import uniforms
+    u = uniforms(11, 2)
+    return 0.3 + 0.4 * u[0], 0.3 + 0.4 * u[1]
+
+
+def _bl_pars(m1, m2):
+    return dict(BH1_mass=m1, BH2_mass=m2, BH1_posn_z=0.5, BH2_posn_z=-0.5, RMAX=7.5, eta=2.0)
+
+
+def _bl_J_gpu(torch, m1, m2, nsteps):
+    from nrpytutorial_amd.runtime import Bundle, Grid, Evolution
+    g = Grid(Bundle("bssn_spherical_sym2"), BL_NXX, **_bl_pars(m1, m2))
+    ev = Evolution(g)
+    ev.initial_data()
+    ev.step(g.find_timestep(0.5), nsteps, use_graph=True)
+    H = g.interior(ev.hamiltonian())
+    J = float(torch.sqrt(torch.mean(H * H)))
+    ev.free_graph()
+    return J
+
+
+def test_bssn_bl_hl2_mass_gradient(torch_cuda):
+    """Gradient of the Hamiltonian-constraint L2 norm after 20 RK4 steps of
+    Brill-Lindquist data (Spherical 64x16x2, RMAX 7.5, masses seeded in [0.3,0.7] at
+    z=+-0.5, CFL 0.5) with respect to the two masses: adjoint (det(gammabar), ghost
+    fill, RHS and Ricci transposes chained through every substep) against central
+    differences h=1e-5 of the GPU evolution and of the C oracle's evolution (CPU),
+    within 1e-6 relative."""
+    torch = torch_cuda
+    import numpy as np
+    from nrpytutorial_amd.runtime import Bundle, Grid, Evolution
+    from nrpytutorial_amd.adjoint import bssn_hl2_mass_gradient
+    from oracle_lib import Oracle, OracleGrid
+    print("case: BSSN BL 64x16x2 d(H-L2)/d(masses) after 20 steps", flush=True)
+    nsteps = 20
+    m1, m2 = _bl_masses()
+    g = Grid(Bundle("bssn_spherical_sym2"), BL_NXX, **_bl_pars(m1, m2))
+    ev = Evolution(g)
+    J, grad = bssn_hl2_mass_gradient(ev, nsteps)
+    h = 1e-5
+    fd = [(_bl_J_gpu(torch, m1 + h, m2, nsteps) - _bl_J_gpu(torch, m1 - h, m2, nsteps)) / (2 * h),
+          (_bl_J_gpu(torch, m1, m2 + h, nsteps) - _bl_J_gpu(torch, m1, m2 - h, nsteps)) / (2 * h)]
+    o = Oracle("bssn_spherical_sym2")
+
+    def J_oracle(a, b_):
+        og = OracleGrid(o, BL_NXX, _bl_pars(a, b_))
+        y = og.zeros(24)
+        og.initial_data(y)
+        og.apply_bcs(y)
+        og.enforce_detg(y)
+        og.mol_step(y, og.find_timestep(0.5), nsteps)
+        H = og.interior(og.hamiltonian(y))
+        return float(np.sqrt(np.mean(H * H)))
+
+    fdo = [(J_oracle(m1 + h, m2) - J_oracle(m1 - h, m2)) / (2 * h),
+           (J_oracle(m1, m2 + h) - J_oracle(m1, m2 - h)) / (2 * h)]
+    rel = max(abs(a - b_) / abs(b_) for a, b_ in zip(grad, fd))
+    relo = max(abs(a - b_) / abs(b_) for a, b_ in zip(grad, fdo))
+    print("J=%.15e adjoint %s FD(GPU) %s rel %.3e; FD(oracle) %s rel %.3e" % (J, grad, fd, rel, fdo, relo))
+    assert rel < 1e-6
+    assert relo < 1e-6
