@@ -259,7 +259,9 @@ def fused_rk4_kernel(name, args_decl, call_args, body_with_writes, loopopts="Int
         if m:
             lines[i] = "const double __rhs_%s = %s;" % (m.group(1), m.group(2))
             vals.append(m.group(1))
-    tail = ["__builtin_amdgcn_sched_barrier(0);"] + ["NRPY_RK4_UPDATE(S, %sGF, __rhs_%s);" % (v, v) for v in vals]
+    tail = (["__builtin_amdgcn_sched_barrier(0);"] +
+            ["NRPY_RK4_PRELOAD(S, %sGF, %d)" % (v, k) for k, v in enumerate(vals)] +
+            ["NRPY_RK4_STORE(S, %d, __rhs_%s);" % (k, v) for k, v in enumerate(vals)])
     # insert inside the body's closing brace, where the __rhs_* values are in scope
     close = max(i for i, l in enumerate(lines) if l.strip() == "}")
     lines[close:close] = tail
@@ -278,9 +280,12 @@ def fused_rk4_kernel(name, args_decl, call_args, body_with_writes, loopopts="Int
     for n in ("Nxx0", "Nxx1", "Nxx2", "Nxx_plus_2NGHOSTS0", "Nxx_plus_2NGHOSTS1", "Nxx_plus_2NGHOSTS2"):
         if not re.search(r"const int %s = " % n, unpack):
             unpack += "  const int %s = params.%s;\n" % (n, n)
-    return ("template <int S>\n__global__ void __launch_bounds__(%s) %s__gpu(%s, double *rk_yn, double *rk_rt, "
-            "double *rk_out, const double dt) {\n%s%s}\n" %
-            (lp.launch_bounds(), name, args_decl, unpack, "".join("  " + l + "\n" for l in kbody.splitlines())))
+    kernel = ("template <int S>\n__global__ void __launch_bounds__(%s) %s__gpu(%s, double *rk_yn, double *rk_rt, "
+              "double *rk_out, const double dt) {\n%s%s}\n" %
+              (lp.launch_bounds(), name, args_decl, unpack, "".join("  " + l + "\n" for l in kbody.splitlines())))
+    if "InteriorPoints" in loopopts:
+        kernel = lp.stage_lds(name, kernel)[0]
+    return kernel
 
 
 _TOKEN = re.compile(r"[A-Za-z_][A-Za-z_0-9]*|\d+\.\d*(?:[eE][-+]?\d+)?|\d+|\S")

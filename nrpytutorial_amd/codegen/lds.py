@@ -1,0 +1,154 @@
+"""Stage the mixed-derivative stencils of an interior-point kernel in LDS.
+
+The BSSN kernels read each input gridfunction at up to 67 stencil points
+(mixed second derivatives are 16-point stencils per coordinate plane). At the
+register pressure of these kernels (one or two waves per SIMD) the compiler
+issues such a stencil's loads one at a time, each followed by a wait for the
+memory round trip. This pass moves the gridfunctions that carry mixed
+derivatives into LDS once per tile, before any other work, so those stencil
+reads become LDS reads with immediate offsets (a fraction of the L2 latency)
+and no address arithmetic.
+
+Layout. A block is a fixed TILE = 16x4x4 tile of interior points, one lane
+per point. For every staged gridfunction the block holds the box
+(16 + 2*NGHOSTS) x (4 + 2H) x (4 + 2H) around the tile in LDS, x fastest,
+where H is the largest stencil offset of the staged reads. The box starts
+NGHOSTS points before the tile in x, so its rows begin at an even index.
+
+Loading. When every box row is 16-byte aligned in global memory (even row
+length Nxx_plus_2NGHOSTS0 and a 16-byte aligned array), each wave copies
+64 x 16 bytes per instruction straight into LDS (``global_load_lds_dwordx4``:
+lane l writes bytes [16 l, 16 l + 16) after the wave's LDS base), so the
+copy uses no registers. Otherwise the block copies element by element
+through registers. Box points outside the array are not loaded; no active
+point reads them. One barrier follows the copy, ahead of the kernel's
+bounds check, so every thread of the block reaches it.
+
+The pass rewrites generated kernel text: the reads it recognises are the
+``const double v = in_gfs[IDX4(GF, i0+a, i1+b, i2+c)];`` statements the
+finite-difference emitter writes. Arithmetic is untouched, so results are
+bit-identical to the unstaged kernel.
+"""
+import re
+
+TILE = (16, 4, 4)
+# LDS bytes a block may use: 160 KiB per CU, and kernels at two waves per SIMD
+# run two 256-thread blocks per CU.
+MAX_LDS_BYTES = 78 * 1024
+
+_READ = re.compile(r"^(\s*const double \w+ = )(in_gfs|auxevol_gfs)\[IDX4\((\w+),\s*i0([+-]\d+)?,\s*i1([+-]\d+)?,"
+                   r"\s*i2([+-]\d+)?\)\];$")
+_BOUNDS = re.compile(r"^\s*if \(i0 >= Nxx0 \+ NGHOSTS \|\| i1 >= Nxx1 \+ NGHOSTS \|\| i2 >= Nxx2 \+ NGHOSTS\) return;$")
+
+
+def _reads(lines):
+    """{gf: (array, set of offsets)} and first-appearance order of the stencil reads."""
+    reads, order = {}, []
+    for line in lines:
+        m = _READ.match(line)
+        if m:
+            gf = m.group(3)
+            if gf not in reads:
+                reads[gf] = (m.group(2), set())
+                order.append(gf)
+            reads[gf][1].add(tuple(int(x) if x else 0 for x in m.group(4, 5, 6)))
+    return reads, order
+
+
+def select(lines, nghosts):
+    """Gridfunctions to stage (those read at a point off two axes at once, i.e. with a
+    mixed derivative), their y/z halo H and LDS box dimensions; None if nothing qualifies
+    or the boxes would not fit."""
+    reads, order = _reads(lines)
+    staged = [g for g in order if any(sum(1 for x in d if x) >= 2 for d in reads[g][1])]
+    if not staged:
+        return None
+    H = max(max(abs(x) for x in d) for g in staged for d in reads[g][1])
+    box = (TILE[0] + 2 * nghosts, TILE[1] + 2 * H, TILE[2] + 2 * H)
+    while staged and len(staged) * box[0] * box[1] * box[2] * 8 > MAX_LDS_BYTES:
+        staged.pop()
+    if not staged:
+        return None
+    return [(g, reads[g][0]) for g in staged], H, box
+
+
+def _prologue(staged, H, box, nghosts):
+    TX, TY, TZ = TILE
+    BX, BY, BZ = box
+    nbox = BX * BY * BZ
+    nchunk = nbox // 2            # 16-byte chunks (BX is even)
+    threads = TX * TY * TZ
+    arrays = sorted(set(a for _, a in staged))
+    aligned = " && ".join("(((size_t)%s & 15) == 0)" % a for a in arrays)
+    s = []
+    s.append("__shared__ __attribute__((aligned(16))) double nrpy_lds[%d][%d];" % (len(staged), nbox))
+    s.append("{")
+    s.append("  /* copy the (%d x %d x %d)-point box around this 16x4x4 tile of %s to LDS */"
+             % (BX, BY, BZ, ", ".join(g[:-2] if g.endswith("GF") else g for g, _ in staged)))
+    s.append("  const int t_ = (int)threadIdx.x + %d * ((int)threadIdx.y + %d * (int)threadIdx.z);" % (TX, TY))
+    s.append("  const int x0_ = nrpy_bx * %d, y0_ = NGHOSTS - %d + nrpy_by * %d, z0_ = NGHOSTS - %d + nrpy_bz * %d;"
+             % (TX, H, TY, H, TZ))
+    s.append("  const size_t ntot_ = (size_t)Nxx_plus_2NGHOSTS0 * Nxx_plus_2NGHOSTS1 * Nxx_plus_2NGHOSTS2;")
+    s.append("  if ((Nxx_plus_2NGHOSTS0 & 1) == 0 && %s) {" % aligned)
+    s.append("    const int w_ = t_ >> 6, l_ = t_ & 63;")
+    s.append("    for (int it_ = 0; it_ < %d; it_++) {" % ((nchunk + threads - 1) // threads))
+    s.append("      const int c_ = (it_ * %d + w_) * 64 + l_;" % (threads // 64))
+    s.append("      const int row_ = c_ / %d, xc_ = c_ - row_ * %d;" % (BX // 2, BX // 2))
+    s.append("      const int z_ = row_ / %d, y_ = row_ - z_ * %d;" % (BY, BY))
+    s.append("      if (c_ < %d && z0_ + z_ < Nxx_plus_2NGHOSTS2 && y0_ + y_ < Nxx_plus_2NGHOSTS1 && "
+             "x0_ + 2 * xc_ + 1 < Nxx_plus_2NGHOSTS0) {" % nchunk)
+    s.append("        const size_t off_ = ((size_t)(z0_ + z_) * Nxx_plus_2NGHOSTS1 + (y0_ + y_)) * Nxx_plus_2NGHOSTS0 + "
+             "x0_ + 2 * xc_;")
+    for q, (g, a) in enumerate(staged):
+        s.append("        __builtin_amdgcn_global_load_lds((const void *)(%s + (size_t)%s * ntot_ + off_), "
+                 "(__attribute__((address_space(3))) void *)&nrpy_lds[%d][(it_ * %d + w_) * 128], 16, 0, 0);"
+                 % (a, g, q, threads // 64))
+    s.append("      }")
+    s.append("    }")
+    s.append("  } else {")
+    s.append("    for (int e_ = t_; e_ < %d; e_ += %d) {" % (nbox, threads))
+    s.append("      const int row_ = e_ / %d, x_ = e_ - row_ * %d;" % (BX, BX))
+    s.append("      const int z_ = row_ / %d, y_ = row_ - z_ * %d;" % (BY, BY))
+    s.append("      if (z0_ + z_ < Nxx_plus_2NGHOSTS2 && y0_ + y_ < Nxx_plus_2NGHOSTS1 && x0_ + x_ < Nxx_plus_2NGHOSTS0) {")
+    s.append("        const size_t off_ = ((size_t)(z0_ + z_) * Nxx_plus_2NGHOSTS1 + (y0_ + y_)) * Nxx_plus_2NGHOSTS0 + "
+             "x0_ + x_;")
+    for q, (g, a) in enumerate(staged):
+        s.append("        nrpy_lds[%d][e_] = %s[(size_t)%s * ntot_ + off_];" % (q, a, g))
+    s.append("      }")
+    s.append("    }")
+    s.append("  }")
+    s.append("}")
+    s.append("__syncthreads();")
+    s.append("const int nrpy_li = (((int)threadIdx.z + %d) * %d + ((int)threadIdx.y + %d)) * %d + (int)threadIdx.x + %d;"
+             % (H, BY, H, BX, nghosts))
+    return s
+
+
+def stage_kernel(kernel_src, nghosts):
+    """Kernel text with its mixed-derivative stencils staged in LDS, or None when the
+    kernel has none (or no recognisable bounds check). The result must be launched
+    with blockDim = TILE."""
+    lines = kernel_src.split("\n")
+    sel = select(lines, nghosts)
+    if sel is None:
+        return None
+    staged, H, box = sel
+    bounds = [k for k, line in enumerate(lines) if _BOUNDS.match(line)]
+    if len(bounds) != 1:
+        return None
+    q_of = {g: q for q, (g, _) in enumerate(staged)}
+    BX, BY, _ = box
+    indent = re.match(r"^(\s*)", lines[bounds[0]]).group(1)
+    out = []
+    for k, line in enumerate(lines):
+        if k == bounds[0]:
+            out += [indent + p for p in _prologue(staged, H, box, nghosts)]
+            out.append(line)
+            continue
+        m = _READ.match(line)
+        if m and m.group(3) in q_of:
+            dx, dy, dz = (int(x) if x else 0 for x in m.group(4, 5, 6))
+            out.append("%snrpy_lds[%d][nrpy_li + %d];" % (m.group(1), q_of[m.group(3)], (dz * BY + dy) * BX + dx))
+            continue
+        out.append(line)
+    return "\n".join(out)

@@ -217,6 +217,8 @@ def _fused_launcher(name, args_sig, call_args, loopopts):
     n = ("params->Nxx0", "params->Nxx1", "params->Nxx2") if rng_interior else \
         ("params->Nxx_plus_2NGHOSTS0", "params->Nxx_plus_2NGHOSTS1", "params->Nxx_plus_2NGHOSTS2")
     kid = bd.lp.kernel_id(name)
+    block = ("dim3(%d, %d, %d)" % tuple(bd.lp.LDS_KERNELS[name]) if name in bd.lp.LDS_KERNELS else
+             "nrpy_block(%d, %d, %d, %d)" % (kid, bd.lp.BLOCK_X, bd.lp.BLOCK_Y, bd.lp.BLOCK_Z))
     cases = "".join(
         "    case %d: hipLaunchKernelGGL(%s__gpu<%d>, grid, block, 0, stream, %s, y_n, rt, out, dt); break;\n"
         % (s, name, s, call_args) for s in range(4))
@@ -225,11 +227,11 @@ def _fused_launcher(name, args_sig, call_args, loopopts):
             "  if (!params || !y_n || !rt || !out) return NRPY_ERR_NULL;\n"
             "  if (S < 0 || S > 3) return NRPY_ERR_PARAM;\n"
             "  if (nrpy_check_params(params)) return NRPY_ERR_SHAPE;\n"
-            "  const dim3 block = nrpy_block(%d, %d, %d, %d);\n"
+            "  const dim3 block = %s;\n"
             "  const dim3 grid(nrpy_nblocks(block, %s, %s, %s));\n"
             "  switch (S) {\n%s  }\n"
             "  return hipGetLastError() == hipSuccess ? 0 : NRPY_ERR_LAUNCH;\n}\n"
-            % (name, args_sig, kid, bd.lp.BLOCK_X, bd.lp.BLOCK_Y, bd.lp.BLOCK_Z, n[0], n[1], n[2], cases))
+            % (name, args_sig, block, n[0], n[1], n[2], cases))
 
 
 # Output groups of the BSSN RHS, one kernel each: fewer live values per kernel. Measured on
@@ -481,6 +483,11 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     args_decl = ("const paramstruct params, const rfm_struct rfmstruct, const double *__restrict__ auxevol_gfs, "
                  "const double *__restrict__ in_gfs")
     rhs_src, fused, group_costs = "", "", []
+    # Large Cartesian grids: the RHS, Ricci and Hamiltonian kernels read their
+    # mixed-derivative stencils from LDS tiles (codegen/lds.py). The small
+    # curvilinear grids of the Brill-Lindquist runs (two points in phi) keep the
+    # plain kernels.
+    bd.lp.LDS_STAGING = (FD_order // 2 + 1) if CoordSystem == "Cartesian" else 0
     for k, grp in enumerate(groups):
         sel = [e for (n, _), e in zip(pairs, evol_rhss) if n in grp]
         body = fin.FD_outputC("returnstring", sel, params="outCverbose=False", upwindcontrolvec=betaU)
@@ -536,6 +543,7 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
                             params="rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
                                    "const REAL *restrict in_gfs, REAL *restrict aux_gfs",
                             body=h_body, loopopts="InteriorPoints,Enable_rfm_precompute")
+    bd.lp.LDS_STAGING = 0
     detg_body = fin.FD_outputC("returnstring", detg_exprs, params="outCverbose=False")
     detg_src = oc.outCfunction(outfile="returnstring", desc="Enforce det(gammabar) = det(gammahat) constraint.",
                                name="enforce_detgammabar_constraint",
@@ -598,6 +606,7 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     meta["detg_vjp_slots"] = [sl[0] for sl in dslots]
     meta["hamiltonian_vjp_slots"] = [sl[0] for sl in hslots]
     meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
+    meta["lds_kernels"] = dict(bd.lp.LDS_KERNELS)
     meta["cost"] = {"rhs_eval": {"flops": sum(c["flops"] for c in group_costs), "groups": group_costs},
                     "Ricci_eval": {"flops": sum(c["flops"] for c in ricci_costs), "groups": ricci_costs},
                     "Hamiltonian_constraint": bd.count_body(h_body),

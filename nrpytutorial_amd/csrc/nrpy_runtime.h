@@ -756,20 +756,31 @@ extern "C" int gridfunction_sumsq(const paramstruct *params, const double *gfs, 
 // (MoLtimestepping/C_Code_Generation.py:266-329, RK4 table at
 // RK_Butcher_Table_Dictionary.py:86-91). 'r' is the RHS at the current point;
 // the arithmetic order matches the reference's generated update loop.
-#define NRPY_RK4_UPDATE(S, gf, rhsval)                                                    \
+//
+// A fused kernel first loads y_n (and the running total) of every output
+// gridfunction with NRPY_RK4_PRELOAD(S, gf, k), then stores all of them with
+// NRPY_RK4_STORE(S, k, rhs). rk_out is rk_yn in the last substep, so the
+// arrays cannot be __restrict__; with the loads interleaved with the stores,
+// the compiler would have to complete each load before the previous store
+// (one memory round trip per output). Loading everything first is the same
+// computation: each thread reads and writes only its own point.
+#define NRPY_RK4_PRELOAD(S, gf, k)                                                        \
+  const size_t nrpy_ix_##k = IDX4(gf, i0, i1, i2);                                        \
+  const double nrpy_y_##k = rk_yn[nrpy_ix_##k];                                           \
+  const double nrpy_r_##k = ((S) == 0) ? 0.0 : rk_rt[nrpy_ix_##k];
+#define NRPY_RK4_STORE(S, k, rhsval)                                                      \
   do {                                                                                    \
-    const size_t _ix = IDX4(gf, i0, i1, i2);                                              \
     const double _r = (rhsval);                                                           \
     if ((S) == 0) {                                                                       \
-      rk_rt[_ix] = _r * dt * (1.0 / 6.0);                                                 \
-      rk_out[_ix] = rk_yn[_ix] + _r * dt * (1.0 / 2.0);                                   \
+      rk_rt[nrpy_ix_##k] = _r * dt * (1.0 / 6.0);                                         \
+      rk_out[nrpy_ix_##k] = nrpy_y_##k + _r * dt * (1.0 / 2.0);                           \
     } else if ((S) == 1) {                                                                \
-      rk_rt[_ix] = rk_rt[_ix] + _r * dt * (1.0 / 3.0);                                    \
-      rk_out[_ix] = rk_yn[_ix] + _r * dt * (1.0 / 2.0);                                   \
+      rk_rt[nrpy_ix_##k] = nrpy_r_##k + _r * dt * (1.0 / 3.0);                            \
+      rk_out[nrpy_ix_##k] = nrpy_y_##k + _r * dt * (1.0 / 2.0);                           \
     } else if ((S) == 2) {                                                                \
-      rk_rt[_ix] = rk_rt[_ix] + _r * dt * (1.0 / 3.0);                                    \
-      rk_out[_ix] = rk_yn[_ix] + _r * dt;                                                 \
+      rk_rt[nrpy_ix_##k] = nrpy_r_##k + _r * dt * (1.0 / 3.0);                            \
+      rk_out[nrpy_ix_##k] = nrpy_y_##k + _r * dt;                                         \
     } else {                                                                              \
-      rk_out[_ix] = rk_yn[_ix] + rk_rt[_ix] + _r * dt * (1.0 / 6.0);                      \
+      rk_out[nrpy_ix_##k] = nrpy_y_##k + nrpy_r_##k + _r * dt * (1.0 / 6.0);              \
     }                                                                                     \
   } while (0)

@@ -39,6 +39,26 @@ KERNEL_IDS = {}
 BATCH_SPECS = {}
 
 
+# Nonzero while a bundle generator emits kernels that should read their
+# mixed-derivative stencils from LDS (codegen/lds.py); the value is NGHOSTS.
+LDS_STAGING = 0
+# kernel name -> fixed block shape of every kernel rewritten that way
+LDS_KERNELS = {}
+
+
+def stage_lds(name, kernel):
+    """(kernel text, launcher block expression or None): the LDS-staged kernel when
+    LDS_STAGING is on and the kernel has mixed-derivative stencils."""
+    if not LDS_STAGING:
+        return kernel, None
+    from .codegen import lds
+    staged = lds.stage_kernel(kernel, LDS_STAGING)
+    if staged is None:
+        return kernel, None
+    LDS_KERNELS[name] = list(lds.TILE)
+    return staged, "dim3(%d, %d, %d)" % lds.TILE
+
+
 def kernel_id(name):
     if name not in KERNEL_IDS:
         KERNEL_IDS[name] = len(KERNEL_IDS)
@@ -206,6 +226,9 @@ def kernel_and_launcher(name, desc, args, preloop, body, loopopts, postloop):
               (desc, launch_bounds(), kname, ", ".join(kargs), unpack,
                "".join("  " + l + "\n" for l in full.splitlines())))
     rng = "interior" if o["range"] == "interior" else "all"
+    block = None
+    if rng == "interior":
+        kernel, block = stage_lds(name, kernel)
     if rng == "interior":
         n0, n1, n2 = "params->Nxx0", "params->Nxx1", "params->Nxx2"
     else:
@@ -217,11 +240,12 @@ def kernel_and_launcher(name, desc, args, preloop, body, loopopts, postloop):
         "  if (!params) return NRPY_ERR_NULL;\n"
         "  if (%s) return NRPY_ERR_NULL;\n"
         "  if (nrpy_check_params(params)) return NRPY_ERR_SHAPE;\n"
-        "  const dim3 block = nrpy_block(%d, %d, %d, %d);\n"
+        "  const dim3 block = %s;\n"
         "  const dim3 grid(nrpy_nblocks(block, %s, %s, %s));\n"
         "  hipLaunchKernelGGL(%s, grid, block, 0, stream, %s);\n"
         "  return hipGetLastError() == hipSuccess ? 0 : NRPY_ERR_LAUNCH;\n"
-        "}\n" % (name, ", ".join(largs), ptr_checks, kid, BLOCK_X, BLOCK_Y, BLOCK_Z, n0, n1, n2, kname,
+        "}\n" % (name, ", ".join(largs), ptr_checks,
+                 block or "nrpy_block(%d, %d, %d, %d)" % (kid, BLOCK_X, BLOCK_Y, BLOCK_Z), n0, n1, n2, kname,
                  ", ".join(call)))
     proto = 'extern "C" int %s(%s, hipStream_t stream);' % (name, ", ".join(largs))
     return kernel, launcher, proto

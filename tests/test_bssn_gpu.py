@@ -41,11 +41,19 @@ def _cart(torch, Nxx, seed=7):
     return b, g, ev, og, y
 
 
-def test_cartesian_ricci_rhs_hamiltonian(torch_cuda):
+# The Cartesian kernels stage mixed-derivative stencils in LDS per 16x4x4 tile. (14,12,10)
+# has an even row length (16-byte aligned rows: direct global->LDS copies) and partial
+# tiles in x and z; (13,9,7) has an odd row length (the copy through registers) and
+# partial tiles in every direction.
+CART_SIZES = [(14, 12, 10), (13, 9, 7)]
+
+
+@pytest.mark.parametrize("Nxx", CART_SIZES)
+def test_cartesian_ricci_rhs_hamiltonian(torch_cuda, Nxx):
     torch = torch_cuda
     from nrpytutorial_amd.runtime import ptr, check
-    print("case: bssn_cartesian Ricci/rhs/H, Nxx=(14,12,10), seed 7", flush=True)
-    b, g, ev, og, y = _cart(torch, (14, 12, 10))
+    print("case: bssn_cartesian Ricci/rhs/H, Nxx=%s, seed 7" % (Nxx,), flush=True)
+    b, g, ev, og, y = _cart(torch, Nxx)
     check(b.lib.Ricci_eval(g.rfm, g.pp(), ptr(ev.y_n), ptr(ev.auxevol), ev.s()), "Ricci_eval")
     rhs = g.empty(len(b.evol))
     check(b.lib.rhs_eval(g.rfm, g.pp(), ptr(ev.auxevol), ptr(ev.y_n), ptr(rhs), ev.s()), "rhs_eval")
@@ -61,12 +69,13 @@ def test_cartesian_ricci_rhs_hamiltonian(torch_cuda):
     assert err_r < TOL and err_rhs < TOL and err_h < TOL
 
 
-def test_cartesian_fused_substep_equals_rhs_plus_update(torch_cuda):
+@pytest.mark.parametrize("Nxx", [(12, 12, 12), (13, 9, 7)])
+def test_cartesian_fused_substep_equals_rhs_plus_update(torch_cuda, Nxx):
     """rhs_eval_rk4_substep(S=0) == RK4 update applied to rhs_eval, at interior points."""
     torch = torch_cuda
     from nrpytutorial_amd.runtime import ptr, check
-    print("case: bssn_cartesian fused substep 0", flush=True)
-    b, g, ev, og, y = _cart(torch, (12, 12, 12))
+    print("case: bssn_cartesian fused substep 0, Nxx=%s" % (Nxx,), flush=True)
+    b, g, ev, og, y = _cart(torch, Nxx)
     check(b.lib.Ricci_eval(g.rfm, g.pp(), ptr(ev.y_n), ptr(ev.auxevol), ev.s()), "Ricci_eval")
     rhs = g.empty(len(b.evol))
     check(b.lib.rhs_eval(g.rfm, g.pp(), ptr(ev.auxevol), ptr(ev.y_n), ptr(rhs), ev.s()), "rhs_eval")
