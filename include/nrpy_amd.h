@@ -160,6 +160,11 @@ int bc_apply_T(const paramstruct *params, const bc_struct *bc, const int NUM_GFS
 int nrpy_lincomb(size_t n, double *out, double a, const double *x, double b, const double *y,
                  hipStream_t stream);
 int nrpy_fill(size_t n, double *out, double v, hipStream_t stream);
+/* RK4 update of the diagonal schedule at every point of n doubles (the update loop of the
+ * reference's RK_MoL.h, MoLtimestepping/C_Code_Generation.py:266-329): substeps 0-2
+ * turn the RHS in k into the next stage input (and accumulate rt), substep 3 updates y_n. */
+int nrpy_rk4_update_all(const int S, size_t n, double *k, double *y_n, double *rt, const double dt,
+                        hipStream_t stream);
 /* acc = (first ? L : acc) + w and, if kbar is not NULL, kbar = a*L + b*w: one pass of the
  * RK4 step adjoint's stage bookkeeping. */
 int nrpy_adjoint_combine(size_t n, double *acc, const double *L, const double *w, double *kbar, double a, double b,

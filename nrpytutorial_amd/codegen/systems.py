@@ -99,6 +99,25 @@ def _mol_glue(has_ricci, has_detg, rhs_args_decl, rhs_call_args):
         "  if (!g) return;\n  (void)hipGraphExecDestroy(g->exec);\n  (void)hipGraphDestroy(g->graph);\n  delete g;\n}\n")
 
 
+def _sommerfeld_glue(fd_order, rhs_call, **defaults):
+    """Sommerfeld radiation condition (CurviBoundaryConditions.py:259-658) and the RK4 step of
+    the Start-to-Finish drivers' BoundaryCondition = "Sommerfeld" branch
+    (Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Cartesian.ipynb:240-246):
+    RHS_string = rhs_eval + apply_bcs_sommerfeld on the RHS, no post-RHS boundary fill, the
+    update at every point. Returns (HIP source, per-gridfunction parameters for the meta)."""
+    from ..CurviBoundaryConditions import CurviBoundaryConditions as cbcs
+    from ..MoLtimestepping import C_Code_Generation as MoL
+    bcs = cbcs.sommerfeld_boundary_condition_class(fd_order=fd_order, **defaults)
+    rhs_string = (rhs_call + "; apply_bcs_sommerfeld(params, xx, bcstruct, NUM_EVOL_GFS, evol_gf_parity, "
+                  "RK_INPUT_GFS, RK_OUTPUT_GFS, stream);")
+    step = MoL.mol_step_function("MoL_step_forward_in_time_sommerfeld", "double *xx[3], const bc_struct *bcstruct",
+                                 rhs_string)
+    meta = {"fd_order": fd_order, "at_inf": list(bcs.vars_at_infinity.values()),
+            "radial_falloff_power": list(bcs.vars_radial_falloff_power.values()),
+            "speed": list(bcs.vars_speed.values())}
+    return "// One RK4 step with the Sommerfeld radiation condition on the RHS.\n" + bcs.hip_source() + step, meta
+
+
 def _mol_batch_glue(n_ricci, n_rhs):
     """MoL_batch_step / MoL_batch_graph_create: one RK4 step of every instance of a batch,
     the diagonal schedule with one launch per kernel per substep for all instances."""
@@ -322,11 +341,13 @@ def scalarwave_cartesian():
                              "const double *auxevol_gfs, const double *in_gfs", "*params, in_gfs", "InteriorPoints")
     id_src = _initial_data_kernel([swid.uu_ID, swid.vv_ID], ["uu", "vv"])
     vjp_src, slots, vouts, vdep = adj.vjp_kernels("rhs_eval", exprs, "", "InteriorPoints", "")
+    som_src, som_meta = _sommerfeld_glue(4, "rhs_eval(params, RK_INPUT_GFS, RK_OUTPUT_GFS, stream)")
     pre, meta = _header(3)
     parts = [pre, bd.coord_host_functions("Cartesian"), '#include "../nrpy_runtime.h"', bd.rfm_struct_code(),
              rhs_src, fused, id_src, vjp_src,
              _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN"),
-             _mol_adjoint_glue(False, vdep, sorted(vouts["EVOL"]) == sorted(gri.gridfunction_lists()[0]))]
+             _mol_adjoint_glue(False, vdep, sorted(vouts["EVOL"]) == sorted(gri.gridfunction_lists()[0])), som_src]
+    meta["sommerfeld"] = som_meta
     meta["vjp_slots"] = [sl[0] for sl in slots]
     meta["cost"] = {"rhs_eval": bd.count_body(body)}
     meta.update({"bundle": "scalarwave_cartesian", "CoordSystem": "Cartesian", "rfm_tables": [],
@@ -370,11 +391,14 @@ def maxwell_cartesian():
                              "const double *auxevol_gfs, const double *in_gfs", "*params, in_gfs", "InteriorPoints")
     id_src = _initial_data_kernel(id_rhss, names)
     coords = bd.coord_host_functions("Cartesian")
+    # the Start-to-Finish Maxwell run's choice: second-order radial derivative, falloff power 3
+    som_src, som_meta = _sommerfeld_glue(2, "rhs_eval(params, RK_INPUT_GFS, RK_OUTPUT_GFS, stream)")
     pre, meta = _header(3)
     vjp_src, slots, vouts, vdep = adj.vjp_kernels("rhs_eval", exprs, "", "InteriorPoints", "")
     parts = [pre, coords, '#include "../nrpy_runtime.h"', bd.rfm_struct_code(), rhs_src, con_src, fused, id_src,
              vjp_src, _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN"),
-             _mol_adjoint_glue(False, vdep, sorted(vouts["EVOL"]) == sorted(gri.gridfunction_lists()[0]))]
+             _mol_adjoint_glue(False, vdep, sorted(vouts["EVOL"]) == sorted(gri.gridfunction_lists()[0])), som_src]
+    meta["sommerfeld"] = som_meta
     meta["vjp_slots"] = [sl[0] for sl in slots]
     meta["cost"] = {"rhs_eval": bd.count_body(body)}
     meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)

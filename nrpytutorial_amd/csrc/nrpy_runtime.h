@@ -65,6 +65,7 @@ struct bc_struct {
   // device copies of the lists (all layers concatenated)
   int *d_ob_dest = nullptr;
   int *d_ob_stride = nullptr;      // FACE . (1, N0, N0*N1)
+  int8_t *d_ob_face = nullptr;     // FACE, 3 per outer point (Sommerfeld radiation condition)
   int *d_ib_dest = nullptr;
   int *d_ib_src = nullptr;
   int8_t *d_ib_parity = nullptr;   // 10 per inner point
@@ -337,6 +338,7 @@ extern "C" void bcstruct_free(bc_struct *bc) {
   if (!bc) return;
   (void)hipFree(bc->d_ob_dest);
   (void)hipFree(bc->d_ob_stride);
+  (void)hipFree(bc->d_ob_face);
   (void)hipFree(bc->d_ib_dest);
   (void)hipFree(bc->d_ib_src);
   (void)hipFree(bc->d_ib_parity);
@@ -389,13 +391,16 @@ extern "C" int bcstruct_create(const paramstruct *params, double *xx[3], bc_stru
   bc_struct *bc = *out;
   *out = nullptr;
   std::vector<int> obd, obs, ibd, ibs;
-  std::vector<int8_t> ibp;
+  std::vector<int8_t> ibp, obf;
   const int N0 = bc->N[0], N1 = bc->N[1];
   bc->ob_off[0] = bc->ib_off[0] = 0;
   for (int gz = 0; gz < NGHOSTS; gz++) {
     for (const outer_bc &o : bc->outer[gz]) {
       obd.push_back(o.outer_bc_dest_pt.i0 + N0 * (o.outer_bc_dest_pt.i1 + N1 * o.outer_bc_dest_pt.i2));
       obs.push_back(o.FACEi0 + N0 * (o.FACEi1 + N1 * o.FACEi2));
+      obf.push_back(o.FACEi0);
+      obf.push_back(o.FACEi1);
+      obf.push_back(o.FACEi2);
     }
     for (const inner_bc &n : bc->inner[gz]) {
       ibd.push_back(n.inner_bc_dest_pt.i0 + N0 * (n.inner_bc_dest_pt.i1 + N1 * n.inner_bc_dest_pt.i2));
@@ -407,6 +412,7 @@ extern "C" int bcstruct_create(const paramstruct *params, double *xx[3], bc_stru
   }
   bool ok = nrpy_upload(&bc->d_ob_dest, obd.data(), obd.size()) &&
             nrpy_upload(&bc->d_ob_stride, obs.data(), obs.size()) &&
+            nrpy_upload(&bc->d_ob_face, obf.data(), obf.size()) &&
             nrpy_upload(&bc->d_ib_dest, ibd.data(), ibd.size()) &&
             nrpy_upload(&bc->d_ib_src, ibs.data(), ibs.size()) &&
             nrpy_upload(&bc->d_ib_parity, ibp.data(), ibp.size()) &&
@@ -632,6 +638,40 @@ __global__ void nrpy_lincomb_kernel(size_t n, double *__restrict__ out, double a
                                     double b, const double *__restrict__ y) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     out[i] = y ? a * x[i] + b * y[i] : a * x[i];
+}
+
+// RK4 update of the diagonal schedule at every point (interior and ghost) of
+// NUM_EVOL_GFS gridfunctions, the loop of the reference's generated RK_MoL.h
+// (MoLtimestepping/C_Code_Generation.py:266-329). k holds the RHS on entry and
+// the next stage's input on exit (substeps 0-2); substep 3 updates y_n.
+__global__ void nrpy_rk4_update_all_kernel(int S, size_t n, double *k, double *y_n, double *rt, double dt) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double ki = k[i];
+    if (S == 0) {
+      rt[i] = ki * dt * (1.0 / 6.0);
+      k[i] = y_n[i] + ki * dt * (1.0 / 2.0);
+    } else if (S == 1) {
+      rt[i] = rt[i] + ki * dt * (1.0 / 3.0);
+      k[i] = y_n[i] + ki * dt * (1.0 / 2.0);
+    } else if (S == 2) {
+      rt[i] = rt[i] + ki * dt * (1.0 / 3.0);
+      k[i] = y_n[i] + ki * dt;
+    } else {
+      y_n[i] = y_n[i] + rt[i] + ki * dt * (1.0 / 6.0);
+    }
+  }
+}
+
+extern "C" int nrpy_rk4_update_all(const int S, size_t n, double *k, double *y_n, double *rt, const double dt,
+                                   hipStream_t stream) {
+  if (!k || !y_n || !rt) return NRPY_ERR_NULL;
+  if (S < 0 || S > 3) return NRPY_ERR_PARAM;
+  if (n == 0) return NRPY_OK;
+  const int B = 256;
+  size_t nb = (n + B - 1) / B;
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(nrpy_rk4_update_all_kernel, dim3((unsigned)nb), dim3(B), 0, stream, S, n, k, y_n, rt, dt);
+  return hipGetLastError() == hipSuccess ? NRPY_OK : NRPY_ERR_LAUNCH;
 }
 
 __global__ void nrpy_fill_kernel(size_t n, double *__restrict__ out, double v) {

@@ -72,6 +72,13 @@ PROTOTYPES = {
     "MoL_graph_launch": (_I, [_VP, _I, _VP]),
     "MoL_graph_free": (None, [_VP]),
     "initial_data": (_I, [_VP, _DPP, _VP, _VP]),
+    "nrpy_rk4_update_all": (_I, [_I, ctypes.c_size_t, _VP, _VP, _VP, _D, _VP]),
+}
+# Sommerfeld radiation condition (bundles built with it): apply_bcs_sommerfeld(params, xx, bc,
+# NUM_GFS, gfs_parity, gfs, rhs_gfs, stream) and the RK4 step that uses it.
+SOMMERFELD_PROTOTYPES = {
+    "apply_bcs_sommerfeld": (_I, [_VP, _DPP, _VP, _I, _VP, _VP, _VP, _VP]),
+    "MoL_step_forward_in_time_sommerfeld": (_I, [_VP, _DPP, _VP, _VP, _VP, _VP, _VP, _D, _VP]),
 }
 KERNEL_PROTOTYPES = {
     # reference-style kernels: (params..., stream)
@@ -116,6 +123,11 @@ class Bundle:
             f = getattr(self.lib, sym)
             f.restype = res
             f.argtypes = args
+        if "sommerfeld" in self.meta:
+            for sym, (res, args) in SOMMERFELD_PROTOTYPES.items():
+                f = getattr(self.lib, sym)
+                f.restype = res
+                f.argtypes = args
         family = self.meta.get("family", name.split("_")[0])
         for sym, args in KERNEL_PROTOTYPES.get(family, {}).items():
             f = getattr(self.lib, sym)
@@ -314,6 +326,20 @@ class Evolution:
                                                         ptr(self.k_odd), ptr(self.k_even), ptr(self.rt),
                                                         ptr(self.auxevol), float(dt), self.s()),
                       "MoL_step_forward_in_time")
+
+    def apply_bcs_sommerfeld(self, gfs, rhs_gfs):
+        """Sommerfeld radiation condition on rhs_gfs at every ghost point (bundles built with it)."""
+        check(self.lib.apply_bcs_sommerfeld(self.g.pp(), self.g.xx_dev_ptrs, self.g.bc, len(self.g.bundle.evol),
+                                            self.g.bundle.evol_parity, ptr(gfs), ptr(rhs_gfs), self.s()),
+              "apply_bcs_sommerfeld")
+
+    def step_sommerfeld(self, dt, nsteps=1):
+        """nsteps RK4 steps with the Sommerfeld condition on the RHS and the update at every point."""
+        for _ in range(nsteps):
+            check(self.lib.MoL_step_forward_in_time_sommerfeld(self.g.pp(), self.g.xx_dev_ptrs, self.g.bc,
+                                                               ptr(self.y_n), ptr(self.k_odd), ptr(self.k_even),
+                                                               ptr(self.rt), float(dt), self.s()),
+                  "MoL_step_forward_in_time_sommerfeld")
 
     def hamiltonian(self, gfs=None):
         gfs = self.y_n if gfs is None else gfs

@@ -279,6 +279,39 @@ def coord_functions(CoordSystem):
     return s
 
 
+def sommerfeld_c(CoordSystem, fd_order, at_inf=0.0, power=3.0, speed=1.0):
+    """Per-bundle part of the Sommerfeld oracle: the Start-to-Finish drivers' parameters
+    and r, d f/d r = sum_i fdD_i dx^i/dr, with dx^i/dr = sum_j (d xx^i/d X^j) X^j / r
+    through the Cartesian coordinates X (chain rule; X^j/r is the radial unit vector)."""
+    par.set_parval_from_str("reference_metric::CoordSystem", CoordSystem)
+    rfm.reference_metric()
+    X = [sp.sympify(rfm.xx_to_Cart[d]) for d in range(3)]
+    J = sp.Matrix(3, 3, lambda i, j: sp.diff(X[i], rfm.xx[j]))
+    Jinv = sp.simplify(J.inv())
+    r = sp.sqrt(sum(x**2 for x in X))
+    fdD = sp.symbols("fdD0 fdD1 fdD2", real=True)
+    dfdr = sum(fdD[i] * sum(Jinv[i, j] * X[j] for j in range(3)) for i in range(3)) / r
+    rep, red = sp.cse([r, dfdr], sp.numbered_symbols("tmp_"))
+    lines = ["const double fdD0 = fdD_[0], fdD1 = fdD_[1], fdD2 = fdD_[2];"]
+    lines += ["const double %s = %s;" % (a, cexpr(b)) for a, b in rep]
+    lines += ["*r = %s;" % cexpr(red[0]), "*dfdr = %s;" % cexpr(red[1])]
+    body = "\n".join(lines)
+    n = len(gri.gridfunction_lists()[0])
+    arr = lambda v: "{" + ", ".join([repr(float(v))] * n) + "}"
+    return ("#define ORACLE_HAS_SOMMERFELD 1\n"
+            "static const int oracle_sommerfeld_fd_order = %d;\n"
+            "static const double oracle_sommerfeld_at_inf[NUM_EVOL_GFS] = %s;\n"
+            "static const double oracle_sommerfeld_power[NUM_EVOL_GFS] = %s;\n"
+            "static const double oracle_sommerfeld_speed[NUM_EVOL_GFS] = %s;\n"
+            "int oracle_sommerfeld_settings(double *out) { out[0] = oracle_sommerfeld_fd_order; "
+            "out[1] = oracle_sommerfeld_at_inf[0]; out[2] = oracle_sommerfeld_power[0]; "
+            "out[3] = oracle_sommerfeld_speed[0]; return NUM_EVOL_GFS; }\n"
+            "static void oracle_r_and_dfdr(const paramstruct *params, double xx0, double xx1, double xx2, "
+            "const double fdD_[3], double *r, double *dfdr) {\n"
+            % (fd_order, arr(at_inf), arr(power), arr(speed))
+            + "".join("  " + l + "\n" for l in unpack_params(body) + lines) + "}\n")
+
+
 def header(NGHOSTS):
     evol, aux, auxevol = gri.gridfunction_lists()
     s = '#include "../oracle_runtime_pre.h"\n#define NGHOSTS %d\n' % NGHOSTS
@@ -333,7 +366,8 @@ def scalarwave_cartesian():
     rhs = point_body([lhrh("rhs_gfs[IDX4S(UUGF,i0,i1,i2)]", sw.uu_rhs), lhrh("rhs_gfs[IDX4S(VVGF,i0,i1,i2)]", sw.vv_rhs)])
     idl = ["in_gfs[IDX4S(UUGF,i0,i1,i2)] = %s;" % cexpr(swid.uu_ID),
            "in_gfs[IDX4S(VVGF,i0,i1,i2)] = %s;" % cexpr(swid.vv_ID)]
-    parts = [header(3), paramstruct_c(), coord_functions("Cartesian"), '#include "../oracle_runtime.h"',
+    coords = coord_functions("Cartesian")
+    parts = [header(3), paramstruct_c(), coords, sommerfeld_c("Cartesian", 4), '#include "../oracle_runtime.h"',
              cfunc("oracle_rhs_eval", "const paramstruct *params, double *xx[3], const double *auxevol_gfs, "
                    "const double *in_gfs, double *rhs_gfs", rhs, "interior",
                    "ScalarWave RHS (ScalarWave/ScalarWave_RHSs.py:46-71)"),
@@ -398,7 +432,7 @@ def maxwell_cartesian():
     idl = ["const double %s = %s;" % (a, cexpr(b)) for a, b in rep]
     idl += ["in_gfs[IDX4S(%sGF,i0,i1,i2)] = %s;" % (n.upper(), cexpr(e)) for n, e in zip(sorted(idv), red)]
     coords = coord_functions("Cartesian")
-    parts = [header(3), paramstruct_c(), coords, '#include "../oracle_runtime.h"',
+    parts = [header(3), paramstruct_c(), coords, sommerfeld_c("Cartesian", 2), '#include "../oracle_runtime.h"',
              cfunc("oracle_rhs_eval", "const paramstruct *params, double *xx[3], const double *auxevol_gfs, "
                    "const double *in_gfs, double *rhs_gfs", rhs, "interior",
                    "vacuum Maxwell System II RHS (Maxwell/VacuumMaxwell_Flat_Evol_Cartesian.py:84-112)"),

@@ -58,3 +58,46 @@ void oracle_MoL_step(const paramstruct *params, double *xx[3], const o_bcstruct 
   }
   o_post(params, xx, bc, evol_parity, y_n_gfs);
 }
+
+#ifdef ORACLE_HAS_SOMMERFELD
+/* One RK4 step of the Start-to-Finish drivers' Sommerfeld branch
+ * (Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Cartesian.ipynb:240-246):
+ * RHS_string = rhs_eval + apply_bcs_sommerfeld, post_RHS_string empty; the update loop
+ * runs over all points, so ghost values evolve with the radiation-condition RHS. */
+void oracle_MoL_step_sommerfeld(const paramstruct *params, double *xx[3], const o_bcstruct *bc,
+                                const int8_t *evol_parity, double *y_n_gfs, double *k_odd_gfs, double *k_even_gfs,
+                                double *y_nplus1_running_total_gfs, const double dt) {
+  const size_t N = (size_t)params->Nxx_plus_2NGHOSTS0*params->Nxx_plus_2NGHOSTS1*params->Nxx_plus_2NGHOSTS2*NUM_EVOL_GFS;
+  double *rt = y_nplus1_running_total_gfs;
+#define O_SOM(IN, OUT) oracle_apply_bcs_sommerfeld(params, xx, bc, NUM_EVOL_GFS, evol_parity, IN, OUT, \
+    oracle_sommerfeld_fd_order, oracle_sommerfeld_at_inf, oracle_sommerfeld_power, oracle_sommerfeld_speed)
+  oracle_rhs_eval(params, xx, NULL, y_n_gfs, k_odd_gfs);
+  O_SOM(y_n_gfs, k_odd_gfs);
+#pragma omp parallel for
+  for (size_t i = 0; i < N; i++) {
+    rt[i] = k_odd_gfs[i]*dt*(1.0/6.0);
+    k_odd_gfs[i] = y_n_gfs[i] + k_odd_gfs[i]*dt*(1.0/2.0);
+  }
+  oracle_rhs_eval(params, xx, NULL, k_odd_gfs, k_even_gfs);
+  O_SOM(k_odd_gfs, k_even_gfs);
+#pragma omp parallel for
+  for (size_t i = 0; i < N; i++) {
+    rt[i] = rt[i] + k_even_gfs[i]*dt*(1.0/3.0);
+    k_even_gfs[i] = y_n_gfs[i] + k_even_gfs[i]*dt*(1.0/2.0);
+  }
+  oracle_rhs_eval(params, xx, NULL, k_even_gfs, k_odd_gfs);
+  O_SOM(k_even_gfs, k_odd_gfs);
+#pragma omp parallel for
+  for (size_t i = 0; i < N; i++) {
+    rt[i] = rt[i] + k_odd_gfs[i]*dt*(1.0/3.0);
+    k_odd_gfs[i] = y_n_gfs[i] + k_odd_gfs[i]*dt;
+  }
+  oracle_rhs_eval(params, xx, NULL, k_odd_gfs, k_even_gfs);
+  O_SOM(k_odd_gfs, k_even_gfs);
+#pragma omp parallel for
+  for (size_t i = 0; i < N; i++) {
+    y_n_gfs[i] = y_n_gfs[i] + rt[i] + k_even_gfs[i]*dt*(1.0/6.0);
+  }
+#undef O_SOM
+}
+#endif

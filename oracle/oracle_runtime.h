@@ -154,3 +154,90 @@ double oracle_find_timestep(const paramstruct *params, double *xx[3], double CFL
       }
   return dsmin*CFL_FACTOR/1.0;
 }
+
+#ifdef ORACLE_HAS_SOMMERFELD
+/* Sommerfeld radiation condition (CurviBoundaryConditions.py:259-658): d f/d r at a point
+ * from one-sided stencils along directions where the point is on a face (FACE != 0) or
+ * within NGHOSTS of the array edge, centred stencils otherwise, contracted with d x^i/d r
+ * by oracle_r_and_dfdr (generated per coordinate system). Coefficients: one-sided
+ * fd2 (-3/2, 2, -1/2), fd4 (-25/12, 4, -3, 4/3, -1/4), fd6 (-49/20, 6, -15/2, 20/3,
+ * -15/4, 6/5, -1/6); centred fd2 (1/2), fd4 (2/3, 1/12), fd6 (3/4, 3/20, 1/60). */
+static void o_sommerfeld_point(const paramstruct *params, double *xx[3], const double *f, const int8_t face[3],
+                               int i0, int i1, int i2, int fd_order, double *r, double *dfdr) {
+  const int Nxx_plus_2NGHOSTS0 = params->Nxx_plus_2NGHOSTS0;
+  const int Nxx_plus_2NGHOSTS1 = params->Nxx_plus_2NGHOSTS1;
+  const int N[3] = {params->Nxx_plus_2NGHOSTS0, params->Nxx_plus_2NGHOSTS1, params->Nxx_plus_2NGHOSTS2};
+  const double invdx[3] = {params->invdx0, params->invdx1, params->invdx2};
+  static const double one2[3] = {-1.5, 2.0, -0.5};
+  static const double one4[5] = {-25.0/12.0, 4.0, -3.0, 4.0/3.0, -1.0/4.0};
+  static const double one6[7] = {-49.0/20.0, 6.0, -15.0/2.0, 20.0/3.0, -15.0/4.0, 6.0/5.0, -1.0/6.0};
+  const double *one = fd_order == 2 ? one2 : (fd_order == 4 ? one4 : one6);
+  const int ii[3] = {i0, i1, i2};
+  double fdD[3];
+  for (int d = 0; d < 3; d++) {
+    int e[3] = {0, 0, 0};
+    e[d] = 1;
+#define O_F(k) f[IDX3S(i0 + (k)*e[0], i1 + (k)*e[1], i2 + (k)*e[2])]
+    if (abs(face[d]) == 1 || ii[d] + NGHOSTS >= N[d] || ii[d] - NGHOSTS <= 0) {
+      int S = face[d];
+      if (ii[d] + NGHOSTS >= N[d]) S = -1;
+      if (ii[d] - NGHOSTS <= 0) S = +1;
+      double sum = 0.0;
+      for (int k = 0; k <= fd_order; k++) sum = (k == 0) ? one[0]*O_F(0) : sum + one[k]*O_F(k*S);
+      fdD[d] = S*sum*invdx[d];
+    } else if (fd_order == 2) {
+      fdD[d] = (O_F(1) - O_F(-1))*0.5*invdx[d];
+    } else if (fd_order == 4) {
+      const double c1 = 2.0/3.0, c2 = 1.0/12.0;
+      fdD[d] = (-c2*O_F(2) + c1*O_F(1) - c1*O_F(-1) + c2*O_F(-2))*invdx[d];
+    } else {
+      const double c1 = 3.0/4.0, c2 = 3.0/20.0, c3 = 1.0/60.0;
+      fdD[d] = (c3*O_F(3) - c2*O_F(2) + c1*O_F(1) - c1*O_F(-1) + c2*O_F(-2) - c3*O_F(-3))*invdx[d];
+    }
+#undef O_F
+  }
+  oracle_r_and_dfdr(params, xx[0][i0], xx[1][i1], xx[2][i2], fdD, r, dfdr);
+}
+
+/* apply_bcs_sommerfeld (CurviBoundaryConditions.py:538-620) in the reference's order:
+ * per gridfunction, per layer from the innermost, outer points in list order, then the
+ * parity copies of the RHS at inner points. Per-gridfunction parameters are arguments. */
+void oracle_apply_bcs_sommerfeld(const paramstruct *params, double *xx[3], const o_bcstruct *bc, int NUM_GFS,
+                                 const int8_t *gfs_parity, const double *gfs, double *rhs_gfs, int fd_order,
+                                 const double *at_inf, const double *falloff_power, const double *speed) {
+  const int Nxx_plus_2NGHOSTS0 = params->Nxx_plus_2NGHOSTS0;
+  const int Nxx_plus_2NGHOSTS1 = params->Nxx_plus_2NGHOSTS1;
+  const int Nxx_plus_2NGHOSTS2 = params->Nxx_plus_2NGHOSTS2;
+  const size_t Ntot = (size_t)Nxx_plus_2NGHOSTS0*Nxx_plus_2NGHOSTS1*Nxx_plus_2NGHOSTS2;
+#pragma omp parallel for
+  for (int gf = 0; gf < NUM_GFS; gf++) {
+    const double *f = gfs + gf*Ntot;
+    double *rhs = rhs_gfs + gf*Ntot;
+    for (int gz = 0; gz < NGHOSTS; gz++) {
+      for (int p = 0; p < bc->num_ob[gz]; p++) {
+        const o_outer *o = &bc->outer[gz][p];
+        const int i0 = o->dest.i0, i1 = o->dest.i1, i2 = o->dest.i2;
+        double r = 1e100, dfdr = 1e100;
+        o_sommerfeld_point(params, xx, f, o->face, i0, i1, i2, fd_order, &r, &dfdr);
+        const double invr = 1./r;
+        const double source_rhs = -speed[gf]*(dfdr + invr*(f[IDX3S(i0,i1,i2)] - at_inf[gf]));
+        rhs[IDX3S(i0,i1,i2)] = source_rhs;
+        if (falloff_power[gf] > 0) {
+          const int j0 = i0 + o->face[0], j1 = i1 + o->face[1], j2 = i2 + o->face[2];
+          double r_off = 1e100, dfdr_off = 1e100;
+          o_sommerfeld_point(params, xx, f, o->face, j0, j1, j2, fd_order, &r_off, &dfdr_off);
+          const double invr_off = 1./r_off;
+          const double extrap_rhs = speed[gf]*(dfdr_off + invr_off*(f[IDX3S(j0,j1,j2)] - at_inf[gf]));
+          const double diff = rhs[IDX3S(j0,j1,j2)] + extrap_rhs;
+          rhs[IDX3S(i0,i1,i2)] += diff*pow(r_off*invr, falloff_power[gf]);
+        }
+      }
+      for (int p = 0; p < bc->num_ib[gz]; p++) {
+        const o_inner *n = &bc->inner[gz][p];
+        rhs[IDX3S(n->dest.i0,n->dest.i1,n->dest.i2)] =
+            n->parity[gfs_parity[gf]] * rhs[IDX3S(n->src.i0,n->src.i1,n->src.i2)];
+      }
+    }
+  }
+}
+#endif

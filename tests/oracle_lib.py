@@ -45,6 +45,14 @@ class Oracle:
             L.oracle_Maxwell_constraints.argtypes = [_VP, _DPP, _VP, _VP]
         if hasattr(L, "oracle_Hamiltonian_constraint"):
             L.oracle_Hamiltonian_constraint.argtypes = [_VP, _DPP, _VP, _VP]
+        self.sommerfeld = None
+        if hasattr(L, "oracle_apply_bcs_sommerfeld"):
+            L.oracle_apply_bcs_sommerfeld.argtypes = [_VP, _DPP, _VP, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP]
+            L.oracle_MoL_step_sommerfeld.argtypes = [_VP, _DPP, _VP, _VP, _VP, _VP, _VP, _VP, _D]
+            out = (ctypes.c_double * 4)()
+            L.oracle_sommerfeld_settings(out)
+            self.sommerfeld = {"fd_order": int(out[0]), "at_inf": out[1], "radial_falloff_power": out[2],
+                               "speed": out[3]}
         self.NGHOSTS = L.oracle_NGHOSTS()
         self.evol = self.meta["evol"]
         self.parity = (ctypes.c_int8 * len(self.evol))(*self.meta["evol_parity"])
@@ -145,6 +153,25 @@ class OracleGrid:
             self.o.lib.oracle_MoL_step(ctypes.byref(self.p), self.ptrs(self.xx), self.bc, self.o.parity,
                                        y_n.ctypes.data, k1.ctypes.data, k2.ctypes.data, rt.ctypes.data,
                                        aux.ctypes.data, dt)
+
+    def apply_bcs_sommerfeld(self, y, rhs, fd_order=None, at_inf=None, power=None, speed=None):
+        """Sommerfeld condition on rhs (in place) at every ghost point; parameters default to
+        the bundle's Start-to-Finish settings, scalars apply to every gridfunction."""
+        st = self.o.sommerfeld
+        n = y.shape[0]
+        arr = lambda v, d: np.full(n, float(d if v is None else v))
+        a, pw, sp = arr(at_inf, st["at_inf"]), arr(power, st["radial_falloff_power"]), arr(speed, st["speed"])
+        self.o.lib.oracle_apply_bcs_sommerfeld(ctypes.byref(self.p), self.ptrs(self.xx), self.bc, n,
+                                               self.o.parity, y.ctypes.data, rhs.ctypes.data,
+                                               int(fd_order or st["fd_order"]), a.ctypes.data, pw.ctypes.data,
+                                               sp.ctypes.data)
+
+    def mol_step_sommerfeld(self, y_n, dt, nsteps=1):
+        k1, k2, rt = np.zeros_like(y_n), np.zeros_like(y_n), np.zeros_like(y_n)
+        for _ in range(nsteps):
+            self.o.lib.oracle_MoL_step_sommerfeld(ctypes.byref(self.p), self.ptrs(self.xx), self.bc, self.o.parity,
+                                                  y_n.ctypes.data, k1.ctypes.data, k2.ctypes.data, rt.ctypes.data,
+                                                  dt)
 
     def interior(self, a):
         G = self.o.NGHOSTS

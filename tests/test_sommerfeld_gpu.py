@@ -1,0 +1,80 @@
+"""Sommerfeld radiation condition on the GPU against the C oracle, through the
+bundles' C ABI (scalar wave with fourth-order radial derivatives, vacuum
+Maxwell with the second-order ones of the Start-to-Finish run):
+
+* apply_bcs_sommerfeld after rhs_eval: every point of the RHS to 1e-12 of
+  each gridfunction's max norm;
+* MoL_step_forward_in_time_sommerfeld (the drivers' Sommerfeld branch: the
+  condition on the RHS, the update at every point, no ghost fill): the state
+  at every point after 10 RK4 steps to 1e-10.
+
+The state is the bundle's initial data plus a smooth, non-radial perturbation
+that differs per gridfunction, so the one-sided and centred radial-derivative
+stencils and the falloff term all see non-trivial data.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_maxnorm_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+TOL_EVOL = 1e-10
+CASES = [("scalarwave_cartesian", dict(xmin=-3.0, xmax=3.0, ymin=-3.0, ymax=3.0, zmin=-3.0, zmax=3.0)),
+         ("maxwell_cartesian", dict(xmin=-4.0, xmax=4.0, ymin=-4.0, ymax=4.0, zmin=-4.0, zmax=4.0, time=0.4))]
+
+
+def _setup(torch, bundle, pars, Nxx):
+    from nrpytutorial_amd.runtime import Bundle, Grid, Evolution
+    from oracle_lib import Oracle, OracleGrid
+    b = Bundle(bundle)
+    g = Grid(b, Nxx, **pars)
+    ev = Evolution(g)
+    og = OracleGrid(Oracle(bundle), Nxx, pars)
+    y = og.zeros(len(b.evol))
+    og.initial_data(y)
+    z, yy, x = np.meshgrid(og.xx[2], og.xx[1], og.xx[0], indexing="ij")
+    for k in range(y.shape[0]):
+        y[k] += 0.05 * (k + 1) * np.sin(0.7 * x + 0.3 * k) * np.cos(0.5 * yy - 0.2) * np.sin(0.9 * z + 0.1 * k)
+    ev.y_n.copy_(torch.from_numpy(y))
+    return b, g, ev, og, y
+
+
+@pytest.mark.parametrize("bundle,pars", CASES, ids=[c[0] for c in CASES])
+def test_apply_bcs_sommerfeld_matches_oracle(torch_cuda, bundle, pars):
+    torch = torch_cuda
+    from nrpytutorial_amd.runtime import ptr, check
+    Nxx = (14, 12, 10)
+    print("case: %s apply_bcs_sommerfeld Nxx=%s" % (bundle, Nxx), flush=True)
+    b, g, ev, og, y = _setup(torch, bundle, pars, Nxx)
+    assert og.o.sommerfeld["fd_order"] == b.meta["sommerfeld"]["fd_order"]
+    rhs = g.empty(len(b.evol))
+    rhs.zero_()
+    check(b.lib.rhs_eval(g.pp(), ptr(ev.y_n), ptr(rhs), ev.s()), "rhs_eval")
+    ev.apply_bcs_sommerfeld(ev.y_n, rhs)
+    torch.cuda.synchronize()
+    rhs_ref = og.rhs_eval(y)
+    og.apply_bcs_sommerfeld(y, rhs_ref)
+    got = rhs.cpu().numpy()
+    err = rel_maxnorm_err(got, rhs_ref, floor_frac=1e-12)
+    print("%s: Sommerfeld RHS at all points rel err %.2e" % (bundle, err))
+    assert err < TOL, err
+    G = og.o.NGHOSTS
+    ghost = np.ones(og.shape, bool)
+    ghost[G:-G, G:-G, G:-G] = False
+    assert np.all(np.isfinite(got)) and np.abs(got[:, ghost]).max() > 0
+
+
+@pytest.mark.parametrize("bundle,pars", CASES, ids=[c[0] for c in CASES])
+def test_mol_sommerfeld_matches_oracle(torch_cuda, bundle, pars):
+    torch = torch_cuda
+    Nxx = (20, 18, 16)
+    print("case: %s MoL_step_forward_in_time_sommerfeld Nxx=%s 10 steps" % (bundle, Nxx), flush=True)
+    b, g, ev, og, y = _setup(torch, bundle, pars, Nxx)
+    dt = 0.5 * g.params.dxx0
+    ev.step_sommerfeld(dt, 10)
+    og.mol_step_sommerfeld(y, dt, 10)
+    torch.cuda.synchronize()
+    err = rel_maxnorm_err(ev.y_n.cpu().numpy(), y, floor_frac=1e-12)
+    print("%s: 10 RK4 steps with Sommerfeld, rel err %.2e" % (bundle, err))
+    assert err < TOL_EVOL, err
