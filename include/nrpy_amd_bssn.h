@@ -53,6 +53,13 @@ int rhs_eval_rk4_substep_g2(const int S, const rfm_struct *rfmstruct, const para
 int Hamiltonian_constraint(const rfm_struct *rfmstruct, const paramstruct *params, const double *in_gfs,
                            double *aux_gfs, hipStream_t stream);
 
+/* Momentum constraint M^i (BSSN/BSSN_constraints.py:87-126, rescaled by ReU) into aux_gfs
+ * MU0..MU2 at interior points: momentum_constraint(params, xx, in_gfs, aux_gfs) of
+ * Tutorial-Start_to_Finish-BSSNCurvilinear-Setting_up_Exact_Initial_Data.ipynb (cell 17),
+ * with the reference metric read from rfm tables. */
+int momentum_constraint(const rfm_struct *rfmstruct, const paramstruct *params, const double *in_gfs,
+                        double *aux_gfs, hipStream_t stream);
+
 /* Replaces enforce_detgammabar_constraint(const rfm_struct*, const paramstruct*, REAL *in_gfs)
  * (Enforce_Detgammabar_Constraint.py:60-75), at all points, in place. */
 int enforce_detgammabar_constraint(const rfm_struct *rfmstruct, const paramstruct *params, double *in_gfs,

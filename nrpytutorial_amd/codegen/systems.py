@@ -26,6 +26,7 @@ import sympy as sp
 
 from .. import NRPy_param_funcs as par
 from .. import grid as gri
+from .. import indexedexp as ixp
 from .. import reference_metric as rfm
 from .. import finite_difference as fin
 from .. import outputC as oc
@@ -487,6 +488,7 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     Bq.RicciBar__gammabarDD_dHatD__DGammaUDD__DGammaU()
     RbarDD = Bq.RbarDD
     gri.register_gridfunctions("AUX", "H")
+    ixp.register_gridfunctions_for_single_rank1("AUX", "MU")
     bssncon.BSSN_constraints(add_T4UUmunu_source_terms=False)
     print("(BENCH) symbolic expressions %.1fs" % (time.time() - t0), flush=True)
 
@@ -567,6 +569,15 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
                             params="rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
                                    "const REAL *restrict in_gfs, REAL *restrict aux_gfs",
                             body=h_body, loopopts="InteriorPoints,Enable_rfm_precompute")
+    # momentum constraint, as the reference's exact-initial-data driver emits it
+    # (Tutorial-Start_to_Finish-BSSNCurvilinear-Setting_up_Exact_Initial_Data.ipynb, cell 17)
+    mu_body = fin.FD_outputC("returnstring", [lhrh(lhs=gri.gfaccess("aux_gfs", "MU%d" % i), rhs=bssncon.MU[i])
+                                              for i in range(3)], params="outCverbose=False")
+    mu_src = oc.outCfunction(outfile="returnstring", desc="Evaluate the momentum constraint",
+                             name="momentum_constraint",
+                             params="rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
+                                    "const REAL *restrict in_gfs, REAL *restrict aux_gfs",
+                             body=mu_body, loopopts="InteriorPoints,Enable_rfm_precompute")
     bd.lp.LDS_STAGING = 0
     detg_body = fin.FD_outputC("returnstring", detg_exprs, params="outCverbose=False")
     detg_src = oc.outCfunction(outfile="returnstring", desc="Enforce det(gammabar) = det(gammahat) constraint.",
@@ -618,7 +629,7 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     batch_src, batch_table = bt.batched_kernels(batch_names)
     pre, meta = _header(FD_order // 2 + 1)
     parts = [pre, bd.coord_host_functions(CoordSystem), '#include "../nrpy_runtime.h"', rfm_code,
-             ricci_src, rhs_src, fused, h_src, detg_src, id_src, id_dmass_src, vjp_src, rvjp_src, dvjp_src,
+             ricci_src, rhs_src, fused, h_src, mu_src, detg_src, id_src, id_dmass_src, vjp_src, rvjp_src, dvjp_src,
              hvjp_src,
              _mol_glue(True, True, None, "rfmstruct, params, auxevol_gfs, IN"),
              bt.instance_struct(), batch_table, '#include "../nrpy_batch.h"', batch_src,
@@ -634,10 +645,12 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     meta["cost"] = {"rhs_eval": {"flops": sum(c["flops"] for c in group_costs), "groups": group_costs},
                     "Ricci_eval": {"flops": sum(c["flops"] for c in ricci_costs), "groups": ricci_costs},
                     "Hamiltonian_constraint": bd.count_body(h_body),
+                    "momentum_constraint": bd.count_body(mu_body),
                     "enforce_detgammabar_constraint": bd.count_body(detg_body)}
     meta.update({"bundle": bundle_name, "CoordSystem": CoordSystem, "symmetry_axes": symmetry_axes,
                  "rfm_tables": tables, "has_ricci": True, "has_detg": True,
-                 "kernels": ["Ricci_eval", "rhs_eval", "Hamiltonian_constraint", "enforce_detgammabar_constraint",
+                 "kernels": ["Ricci_eval", "rhs_eval", "Hamiltonian_constraint", "momentum_constraint",
+                             "enforce_detgammabar_constraint",
                              "initial_data"]})
     return bd.write_bundle(bundle_name, parts, meta)
 

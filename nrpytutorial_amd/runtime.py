@@ -87,6 +87,7 @@ KERNEL_PROTOTYPES = {
     "bssn": {"rhs_eval": [_VP, _VP, _VP, _VP, _VP, _VP], "Ricci_eval": [_VP, _VP, _VP, _VP, _VP],
              "rhs_eval_vjp": [_VP] * 9, "Ricci_eval_vjp": [_VP] * 9,
              "Hamiltonian_constraint": [_VP, _VP, _VP, _VP, _VP],
+             "momentum_constraint": [_VP, _VP, _VP, _VP, _VP],
              "enforce_detgammabar_constraint": [_VP, _VP, _VP, _VP]},
     "maxwell": {"rhs_eval": [_VP, _VP, _VP, _VP], "Maxwell_constraints": [_VP, _VP, _VP, _VP]},
 }
@@ -340,6 +341,14 @@ class Evolution:
                                                                ptr(self.y_n), ptr(self.k_odd), ptr(self.k_even),
                                                                ptr(self.rt), float(dt), self.s()),
                   "MoL_step_forward_in_time_sommerfeld")
+
+    def momentum(self, gfs=None):
+        """The momentum constraint M^i (rescaled, three gridfunctions) at interior points."""
+        gfs = self.y_n if gfs is None else gfs
+        check(self.lib.momentum_constraint(self.g.rfm, self.g.pp(), ptr(gfs), ptr(self.aux), self.s()),
+              "momentum_constraint")
+        a = self.g.bundle.aux
+        return self.aux[a.index("MU0"):a.index("MU2") + 1]
 
     def hamiltonian(self, gfs=None):
         gfs = self.y_n if gfs is None else gfs

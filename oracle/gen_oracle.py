@@ -474,6 +474,8 @@ def bssn(CoordSystem, symmetry_axes, name):
     par.set_parval_from_str("BSSN.BSSN_quantities::LeaveRicciSymbolic", "False")
     Bq.RicciBar__gammabarDD_dHatD__DGammaUDD__DGammaU()
     gri.register_gridfunctions("AUX", "H")
+    from nrpytutorial_amd import indexedexp as ixp
+    ixp.register_gridfunctions_for_single_rank1("AUX", "MU")
     bssncon.BSSN_constraints()
     ev = {"alpha": gaugerhs.alpha_rhs, "cf": rhs.cf_rhs, "trK": rhs.trK_rhs}
     for i in range(3):
@@ -486,6 +488,7 @@ def bssn(CoordSystem, symmetry_axes, name):
     ric_lines = point_body([lhrh("auxevol_gfs[IDX4S(RBARDD%d%dGF,i0,i1,i2)]" % (i, j), Bq.RbarDD[i][j])
                             for i in range(3) for j in range(i, 3)])
     h_lines = point_body([lhrh("aux_gfs[IDX4S(HGF,i0,i1,i2)]", bssncon.H)])
+    mu_lines = point_body([lhrh("aux_gfs[IDX4S(MU%dGF,i0,i1,i2)]" % i, bssncon.MU[i]) for i in range(3)])
     detg_lines = point_body([lhrh(e.lhs.replace("IDX4(", "IDX4S(").replace(", i0, i1, i2", ",i0,i1,i2"), e.rhs)
                              for e in detg])
     idl = []
@@ -502,6 +505,8 @@ def bssn(CoordSystem, symmetry_axes, name):
                    "BSSN + gauge RHSs (BSSN/BSSN_RHSs.py, BSSN/BSSN_gauge_RHSs.py; Two_BHs_Collide.ipynb:436-484)"),
              cfunc("oracle_Hamiltonian_constraint", "const paramstruct *params, double *xx[3], const double *in_gfs, "
                    "double *aux_gfs", h_lines, "interior", "Hamiltonian constraint (BSSN/BSSN_constraints.py:49-80)"),
+             cfunc("oracle_momentum_constraint", "const paramstruct *params, double *xx[3], const double *in_gfs, "
+                   "double *aux_gfs", mu_lines, "interior", "momentum constraint (BSSN/BSSN_constraints.py:87-126)"),
              cfunc("oracle_enforce_detgammabar_constraint", "const paramstruct *params, double *xx[3], double *in_gfs",
                    detg_lines, "all", "det(gammabar) enforcement (BSSN/Enforce_Detgammabar_Constraint.py:16-75)"),
              cfunc("oracle_initial_data", "const paramstruct *params, double *xx[3], double *in_gfs", idl, "all",

@@ -45,6 +45,8 @@ class Oracle:
             L.oracle_Maxwell_constraints.argtypes = [_VP, _DPP, _VP, _VP]
         if hasattr(L, "oracle_Hamiltonian_constraint"):
             L.oracle_Hamiltonian_constraint.argtypes = [_VP, _DPP, _VP, _VP]
+        if hasattr(L, "oracle_momentum_constraint"):
+            L.oracle_momentum_constraint.argtypes = [_VP, _DPP, _VP, _VP]
         self.sommerfeld = None
         if hasattr(L, "oracle_apply_bcs_sommerfeld"):
             L.oracle_apply_bcs_sommerfeld.argtypes = [_VP, _DPP, _VP, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP]
@@ -130,6 +132,13 @@ class OracleGrid:
         self.o.lib.oracle_Hamiltonian_constraint(ctypes.byref(self.p), self.ptrs(self.xx), y.ctypes.data,
                                                  aux.ctypes.data)
         return aux[0]
+
+    def momentum(self, y):
+        """M^0..M^2 (the AUX gridfunctions after H)."""
+        aux = self.zeros(4)
+        self.o.lib.oracle_momentum_constraint(ctypes.byref(self.p), self.ptrs(self.xx), y.ctypes.data,
+                                              aux.ctypes.data)
+        return aux[1:4]
 
     def maxwell_constraints(self, y):
         aux = self.zeros(2)

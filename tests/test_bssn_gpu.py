@@ -2,8 +2,8 @@
 
 * Cartesian reference metric, seeded flat + Gaussian state (the 192^3 RHS
   benchmark input at a size the oracle finishes in seconds): Ricci_eval,
-  rhs_eval, Hamiltonian_constraint agree to 1e-12 of each gridfunction's max
-  norm (interior points).
+  rhs_eval, Hamiltonian_constraint and momentum_constraint agree to 1e-12 of
+  each gridfunction's max norm (interior points).
 * Spherical, symmetry axis 2, Brill-Lindquist head-on data (m1=m2=0.5 at
   z=+-0.5, RMAX=7.5): initial data at all points, boundary conditions +
   det(gammabar) enforcement, and an RK4 evolution whose state and
@@ -65,8 +65,11 @@ def test_cartesian_ricci_rhs_hamiltonian(torch_cuda, Nxx):
     err_rhs = rel_maxnorm_err(g.interior(rhs.cpu().numpy()), og.interior(rhs_ref))
     H_ref = og.hamiltonian(y)
     err_h = rel_maxnorm_err(g.interior(H.cpu().numpy())[None], og.interior(H_ref)[None])
-    print("Ricci %.2e rhs %.2e H %.2e" % (err_r, err_rhs, err_h))
-    assert err_r < TOL and err_rhs < TOL and err_h < TOL
+    MU = ev.momentum()
+    torch.cuda.synchronize()
+    err_mu = rel_maxnorm_err(g.interior(MU.cpu().numpy()), og.interior(og.momentum(y)))
+    print("Ricci %.2e rhs %.2e H %.2e MU %.2e" % (err_r, err_rhs, err_h, err_mu))
+    assert err_r < TOL and err_rhs < TOL and err_h < TOL and err_mu < TOL
 
 
 @pytest.mark.parametrize("Nxx", [(12, 12, 12), (13, 9, 7)])
