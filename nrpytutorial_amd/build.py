@@ -115,7 +115,7 @@ def build_all(names=None, oracle=True, force=False):
 
 
 ORACLE_BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2", "scalarwave_sinhsphericalv2",
-                  "scalarwave_sinhcylindricalv2", "maxwell_cartesian")
+                  "scalarwave_sinhcylindricalv2", "maxwell_cartesian", "scalarwave_symtp")
 
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("-")]

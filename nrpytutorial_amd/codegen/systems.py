@@ -36,7 +36,8 @@ from . import adjoint as adj
 from . import batch as bt
 
 CURVI_SCALARWAVE = {"scalarwave_sinhsphericalv2": "SinhSphericalv2",
-                    "scalarwave_sinhcylindricalv2": "SinhCylindricalv2"}
+                    "scalarwave_sinhcylindricalv2": "SinhCylindricalv2",
+                    "scalarwave_symtp": "SymTP"}
 BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2", "maxwell_cartesian") + \
     tuple(CURVI_SCALARWAVE)
 

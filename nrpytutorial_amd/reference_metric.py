@@ -171,6 +171,53 @@ def reference_metric(SymPySimplifyExpressions=True, enable_compute_hatted_quanti
                        [-sp.sin(PHICYL), sp.cos(PHICYL), sp.sympify(0)],
                        [sp.sympify(0), sp.sympify(0), sp.sympify(1)]]
 
+    elif CoordSystem in ("SymTP", "SinhSymTP"):
+        # Prolate-spheroidal ("symmetric two-puncture") coordinates (reference_metric.py:306-393):
+        # rho = A sin(xx1), z = sqrt(A^2 + b^2) cos(xx1), phi = xx2, with A = xx0 (SymTP) or a
+        # sinh map of xx0 to [0, AMAX] (SinhSymTP). The foci sit at z = +-bScale.
+        bScale, SINHWAA, AMAX = par.Cparameters("REAL", thismodule, ["bScale", "SINHWAA", "AMAX"], [0.5, 0.2, 10.0])
+        xxmin = [sp.sympify(0), sp.sympify(0), -M_PI]
+        xxmax = [AMAX, M_PI, M_PI]
+        AA = xx[0]
+        if CoordSystem == "SinhSymTP":
+            xxmax[0] = sp.sympify(1)
+            AA = AMAX * (sp.exp(xx[0] / SINHWAA) - sp.exp(-xx[0] / SINHWAA)) / (sp.exp(1 / SINHWAA) - sp.exp(-1 / SINHWAA))
+        th, ph = xx[1], xx[2]
+        var1 = sp.sqrt(AA ** 2 + (bScale * sp.sin(th)) ** 2)
+        var2 = sp.sqrt(AA ** 2 + bScale ** 2)
+        rho, z = AA * sp.sin(th), var2 * sp.cos(th)
+        xx_to_Cart[0] = AA * sp.sin(th) * sp.cos(ph)
+        xx_to_Cart[1] = AA * sp.sin(th) * sp.sin(ph)
+        xx_to_Cart[2] = z
+        xxSph[0] = sp.sqrt(rho ** 2 + z ** 2)
+        xxSph[1] = sp.acos(z / xxSph[0])
+        xxSph[2] = ph
+        if CoordSystem == "SymTP":
+            # inverse map: A^2 and cos^2(xx1) are the roots of a quadratic in terms of the
+            # spherical radius and polar angle of the Cartesian point
+            rS = sp.sqrt(Cartx ** 2 + Carty ** 2 + Cartz ** 2)
+            thS = sp.acos(Cartz / rS)
+            disc = sp.sqrt(bScale ** 4 + 2 * bScale ** 2 * rS ** 2 + rS ** 4 - 4 * bScale ** 2 * rS ** 2 * sp.cos(thS) ** 2)
+            Cart_to_xx[0] = sp.sqrt(-bScale ** 2 + rS ** 2 + disc) * M_SQRT1_2
+            Cart_to_xx[1] = sp.acos(sp.sign(Cartz) * (sp.sqrt(1 + rS ** 2 / bScale ** 2 - disc / bScale ** 2) * M_SQRT1_2))
+            Cart_to_xx[2] = sp.atan2(Carty, Cartx)
+        scalefactor_orthog[0] = sp.diff(AA, xx[0]) * var1 / var2
+        scalefactor_orthog[1] = var1
+        scalefactor_orthog[2] = AA * sp.sin(th)
+        _closed_form["f0_of_xx0"] = AA
+        _closed_form["f1_of_xx1"] = sp.sin(th)
+        _closed_form["f2_of_xx0_xx1"] = var1
+        _closed_form["f3_of_xx0"] = var2
+        f0, f1, f2, f3 = (_funcform[k] for k in ("f0_of_xx0", "f1_of_xx1", "f2_of_xx0_xx1", "f3_of_xx0"))
+        scalefactor_orthog_funcform[0] = sp.diff(f0, xx[0]) * f2 / f3
+        scalefactor_orthog_funcform[1] = f2
+        scalefactor_orthog_funcform[2] = f0 * f1
+        UnitVectors = [[sp.sin(th) * sp.cos(ph) * var2 / var1, sp.sin(th) * sp.sin(ph) * var2 / var1,
+                        AA * sp.cos(th) / var1],
+                       [AA * sp.cos(th) * sp.cos(ph) / var1, AA * sp.cos(th) * sp.sin(ph) / var1,
+                        -sp.sin(th) * var2 / var1],
+                       [-sp.sin(ph), sp.cos(ph), sp.sympify(0)]]
+
     elif CoordSystem == "Cartesian":
         par.Cparameters("REAL", thismodule, ["xmin", "xmax", "ymin", "ymax", "zmin", "zmax"],
                         [-10.0, 10.0, -10.0, 10.0, -10.0, 10.0])

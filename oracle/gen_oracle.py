@@ -518,7 +518,8 @@ def bssn(CoordSystem, symmetry_axes, name):
 
 
 CURVI_SCALARWAVE = {"scalarwave_sinhsphericalv2": "SinhSphericalv2",
-                    "scalarwave_sinhcylindricalv2": "SinhCylindricalv2"}
+                    "scalarwave_sinhcylindricalv2": "SinhCylindricalv2",
+                    "scalarwave_symtp": "SymTP"}
 
 
 if __name__ == "__main__":

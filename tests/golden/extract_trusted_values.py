@@ -53,7 +53,9 @@ KEEP = {
             "rfm_SinhSphericalv2__reference_metric__True__globals",
             "rfm_Cylindrical__reference_metric__True__globals",
             "rfm_SinhCylindrical__reference_metric__True__globals",
-            "rfm_SinhCylindricalv2__reference_metric__True__globals"],
+            "rfm_SinhCylindricalv2__reference_metric__True__globals",
+            "rfm_SymTP__reference_metric__True__globals",
+            "rfm_SinhSymTP__reference_metric__True__globals"],
 }
 
 

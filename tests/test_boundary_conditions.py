@@ -109,6 +109,8 @@ def _compare_lists(h, og, G):
     ("bssn_spherical_sym2", (24, 8, 2), {"RMAX": 7.5}),
     ("bssn_cartesian", (12, 10, 8), {}),
     ("scalarwave_cartesian", (6, 5, 4), {}),
+    ("scalarwave_symtp", (12, 8, 10), {}),
+    ("scalarwave_symtp", (16, 12, 4), {"AMAX": 3.0, "bScale": 0.8}),
 ])
 def test_product_bc_lists_equal_oracle(bundle, Nxx, params):
     """Boundary struct built by the product library on the host, entry by entry
@@ -125,6 +127,7 @@ def test_product_bc_lists_equal_oracle(bundle, Nxx, params):
     ("scalarwave_sinhsphericalv2", (16, 8, 12), {}),
     ("bssn_spherical_sym2", (72, 12, 2), {"RMAX": 7.5}),
     ("scalarwave_sinhcylindricalv2", (12, 8, 10), {}),
+    ("scalarwave_symtp", (12, 8, 10), {}),
 ])
 def test_composed_map_invariants(bundle, Nxx, params):
     """Every ghost point appears once, its sources are interior points, and for

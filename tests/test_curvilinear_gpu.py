@@ -54,7 +54,8 @@ def _curvi(bundle, Nxx, seed=1, **extra):
 
 
 @pytest.mark.parametrize("bundle,Nxx", [("scalarwave_sinhsphericalv2", (16, 8, 16)),
-                                        ("scalarwave_sinhcylindricalv2", (12, 10, 14))])
+                                        ("scalarwave_sinhcylindricalv2", (12, 10, 14)),
+                                        ("scalarwave_symtp", (16, 8, 12))])
 def test_curvilinear_initial_data_and_rhs(torch_cuda, bundle, Nxx):
     torch = torch_cuda
     from nrpytutorial_amd.runtime import check, ptr
@@ -84,7 +85,8 @@ def test_curvilinear_initial_data_and_rhs(torch_cuda, bundle, Nxx):
 
 
 @pytest.mark.parametrize("bundle,Nxx", [("scalarwave_sinhsphericalv2", (16, 8, 16)),
-                                        ("scalarwave_sinhcylindricalv2", (12, 10, 14))])
+                                        ("scalarwave_sinhcylindricalv2", (12, 10, 14)),
+                                        ("scalarwave_symtp", (16, 8, 12))])
 def test_curvilinear_rk4_matches_oracle(torch_cuda, bundle, Nxx):
     torch = torch_cuda
     print("case: %s RK4 8 steps" % bundle, flush=True)

@@ -118,3 +118,17 @@ def test_maxwell_fourth_order_convergence():
         r1 = np.log(e[0] / e[1]) / np.log(32 / 24)
         r2 = np.log(e[1] / e[2]) / np.log(48 / 32)
         assert lo < r1 < 4.6 and lo < r2 < 4.6, (e, r1, r2)
+
+
+def test_symtp_fourth_order_convergence():
+    """SymTP (prolate spheroidal, foci at z = +-bScale; 2-D rfm tables in the bundle),
+    AMAX=3: inner half in xx0 after t=0.5, all angles (parity boundaries at the axis
+    and at A=0 are in the region)."""
+    errs = []
+    for n in (8, 12, 16):
+        errs.append(_max_err("scalarwave_symtp", (2 * n, n, 2 * n), 0.5, 1.0,
+                             (slice(None), slice(None), slice(0, n)), AMAX=3.0))
+    r1 = np.log(errs[0] / errs[1]) / np.log(12 / 8)
+    r2 = np.log(errs[1] / errs[2]) / np.log(16 / 12)
+    print(errs, r1, r2)
+    assert 3.4 < r1 < 4.6 and 3.4 < r2 < 4.6, (errs, r1, r2)

@@ -29,7 +29,7 @@ RFM_INIT = "rfm.reference_metric()\n"
 # (trusted entry, setup code, module expression, globals)
 CASES = []
 for cs in ["Cartesian", "Spherical", "SinhSpherical", "SinhSphericalv2", "Cylindrical", "SinhCylindrical",
-           "SinhCylindricalv2"]:
+           "SinhCylindricalv2", "SymTP", "SinhSymTP"]:
     CASES.append(("rfm_%s__reference_metric__True__globals" % cs,
                   'par.set_parval_from_str("reference_metric::CoordSystem", "%s")\nrfm.reference_metric(True)\n' % cs,
                   "rfm", RFM_NAMES))
