@@ -51,9 +51,10 @@ int apply_bcs_sommerfeld(const paramstruct *params, double *xx[3], const bc_stru
  * (Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Cartesian.ipynb:240-246):
  * per substep rhs_eval + apply_bcs_sommerfeld, then the RK4 update at every point; no
  * ghost fill of the state. xx: device coordinate arrays. */
-int MoL_step_forward_in_time_sommerfeld(const paramstruct *params, double *xx[3], const bc_struct *bcstruct,
-                                        double *y_n_gfs, double *k_odd_gfs, double *k_even_gfs,
-                                        double *y_nplus1_running_total_gfs, const double dt, hipStream_t stream);
+int MoL_step_forward_in_time_sommerfeld(const paramstruct *params, const rfm_struct *rfmstruct, double *xx[3],
+                                        const bc_struct *bcstruct, double *y_n_gfs, double *k_odd_gfs,
+                                        double *k_even_gfs, double *y_nplus1_running_total_gfs, const double dt,
+                                        hipStream_t stream);
 
 #ifdef __cplusplus
 }

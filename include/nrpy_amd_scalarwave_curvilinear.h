@@ -18,6 +18,25 @@ int rhs_eval(const rfm_struct *rfmstruct, const paramstruct *params, const doubl
 /* Plane-wave data in the curvilinear coordinates (ScalarWave/InitialData.py:59-88). */
 int initial_data(const paramstruct *params, double *xx[3], double *in_gfs, hipStream_t stream);
 
+/* Sommerfeld radiation condition on the RHS (apply_bcs_sommerfeld,
+ * CurviBoundaryConditions.py:538-620; no stream in the reference): at every outer ghost
+ * point, innermost layer first, rhs = -v [d f/dr + (f - f_inf)/r] plus the falloff term
+ * extrapolated from one point inward; then the parity copies at inner ghost points.
+ * xx: the three device coordinate arrays; gfs: the state rhs_gfs was evaluated from;
+ * gfs_parity: host array of NUM_GFS parity types; NUM_GFS <= NUM_EVOL_GFS. The
+ * per-gridfunction v, f_inf and falloff power are the bundle's (<bundle>.json "sommerfeld"). */
+int apply_bcs_sommerfeld(const paramstruct *params, double *xx[3], const bc_struct *bc, const int NUM_GFS,
+                         const int8_t *gfs_parity, double *gfs, double *rhs_gfs, hipStream_t stream);
+
+/* One RK4 step of the Start-to-Finish drivers' BoundaryCondition = "Sommerfeld" branch
+ * (Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Cartesian.ipynb:240-246):
+ * per substep rhs_eval + apply_bcs_sommerfeld, then the RK4 update at every point; no
+ * ghost fill of the state. xx: device coordinate arrays. */
+int MoL_step_forward_in_time_sommerfeld(const paramstruct *params, const rfm_struct *rfmstruct, double *xx[3],
+                                        const bc_struct *bcstruct, double *y_n_gfs, double *k_odd_gfs,
+                                        double *k_even_gfs, double *y_nplus1_running_total_gfs, const double dt,
+                                        hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,8 +112,8 @@ def _sommerfeld_glue(fd_order, rhs_call, **defaults):
     bcs = cbcs.sommerfeld_boundary_condition_class(fd_order=fd_order, **defaults)
     rhs_string = (rhs_call + "; apply_bcs_sommerfeld(params, xx, bcstruct, NUM_EVOL_GFS, evol_gf_parity, "
                   "RK_INPUT_GFS, RK_OUTPUT_GFS, stream);")
-    step = MoL.mol_step_function("MoL_step_forward_in_time_sommerfeld", "double *xx[3], const bc_struct *bcstruct",
-                                 rhs_string)
+    step = MoL.mol_step_function("MoL_step_forward_in_time_sommerfeld",
+                                 "const rfm_struct *rfmstruct, double *xx[3], const bc_struct *bcstruct", rhs_string)
     meta = {"fd_order": fd_order, "at_inf": list(bcs.vars_at_infinity.values()),
             "radial_falloff_power": list(bcs.vars_radial_falloff_power.values()),
             "speed": list(bcs.vars_speed.values())}
@@ -448,9 +448,11 @@ def scalarwave_curvilinear(CoordSystem, bundle_name, NGHOSTS=3):
     # The EigenCoord's reference metric registers its own C parameters (e.g. RMAX), so
     # the coordinate functions are generated before the paramstruct is declared.
     coords = bd.coord_host_functions(CoordSystem)
+    som_src, som_meta = _sommerfeld_glue(4, "rhs_eval(rfmstruct, params, RK_INPUT_GFS, RK_OUTPUT_GFS, stream)")
     pre, meta = _header(NGHOSTS)
     parts = [pre, coords, '#include "../nrpy_runtime.h"', rfm_code,
-             rhs_src, fused, id_src, _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN")]
+             rhs_src, fused, id_src, _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN"), som_src]
+    meta["sommerfeld"] = som_meta
     meta["cost"] = {"rhs_eval": bd.count_body(body)}
     meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
     meta.update({"bundle": bundle_name, "family": "scalarwave_curvilinear", "CoordSystem": CoordSystem,

@@ -78,7 +78,7 @@ PROTOTYPES = {
 # NUM_GFS, gfs_parity, gfs, rhs_gfs, stream) and the RK4 step that uses it.
 SOMMERFELD_PROTOTYPES = {
     "apply_bcs_sommerfeld": (_I, [_VP, _DPP, _VP, _I, _VP, _VP, _VP, _VP]),
-    "MoL_step_forward_in_time_sommerfeld": (_I, [_VP, _DPP, _VP, _VP, _VP, _VP, _VP, _D, _VP]),
+    "MoL_step_forward_in_time_sommerfeld": (_I, [_VP, _VP, _DPP, _VP, _VP, _VP, _VP, _VP, _D, _VP]),
 }
 KERNEL_PROTOTYPES = {
     # reference-style kernels: (params..., stream)
@@ -337,7 +337,7 @@ class Evolution:
     def step_sommerfeld(self, dt, nsteps=1):
         """nsteps RK4 steps with the Sommerfeld condition on the RHS and the update at every point."""
         for _ in range(nsteps):
-            check(self.lib.MoL_step_forward_in_time_sommerfeld(self.g.pp(), self.g.xx_dev_ptrs, self.g.bc,
+            check(self.lib.MoL_step_forward_in_time_sommerfeld(self.g.pp(), self.g.rfm, self.g.xx_dev_ptrs, self.g.bc,
                                                                ptr(self.y_n), ptr(self.k_odd), ptr(self.k_even),
                                                                ptr(self.rt), float(dt), self.s()),
                   "MoL_step_forward_in_time_sommerfeld")

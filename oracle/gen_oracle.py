@@ -281,16 +281,19 @@ def coord_functions(CoordSystem):
 
 def sommerfeld_c(CoordSystem, fd_order, at_inf=0.0, power=3.0, speed=1.0):
     """Per-bundle part of the Sommerfeld oracle: the Start-to-Finish drivers' parameters
-    and r, d f/d r = sum_i fdD_i dx^i/dr, with dx^i/dr = sum_j (d xx^i/d X^j) X^j / r
-    through the Cartesian coordinates X (chain rule; X^j/r is the radial unit vector)."""
+    and r, d f/d r = sum_i fdD_i dx^i/dr, with dx^i/dr = (dX/dxx^i . X/r) / h_i^2 through
+    the Cartesian coordinates X (chain rule for orthogonal coordinates; X/r is the radial
+    unit vector, h_i = |dX/dxx^i|). The product inverts d(r,th,ph)/d(xx) instead."""
     par.set_parval_from_str("reference_metric::CoordSystem", CoordSystem)
     rfm.reference_metric()
     X = [sp.sympify(rfm.xx_to_Cart[d]) for d in range(3)]
-    J = sp.Matrix(3, 3, lambda i, j: sp.diff(X[i], rfm.xx[j]))
-    Jinv = sp.simplify(J.inv())
+    # The coordinate systems are orthogonal, so the inverse Jacobian's rows are the
+    # tangent vectors dX/dxx^i divided by their squared lengths h_i^2.
+    T = [[sp.diff(X[j], rfm.xx[i]) for j in range(3)] for i in range(3)]
     r = sp.sqrt(sum(x**2 for x in X))
     fdD = sp.symbols("fdD0 fdD1 fdD2", real=True)
-    dfdr = sum(fdD[i] * sum(Jinv[i, j] * X[j] for j in range(3)) for i in range(3)) / r
+    dfdr = sum(fdD[i] * sum(T[i][j] * X[j] for j in range(3)) / sum(T[i][j]**2 for j in range(3))
+               for i in range(3)) / r
     rep, red = sp.cse([r, dfdr], sp.numbered_symbols("tmp_"))
     lines = ["const double fdD0 = fdD_[0], fdD1 = fdD_[1], fdD2 = fdD_[2];"]
     lines += ["const double %s = %s;" % (a, cexpr(b)) for a, b in rep]
@@ -397,7 +400,8 @@ def scalarwave_curvilinear(CoordSystem, name):
     idl = ["const double %s = %s;" % (a, cexpr(b)) for a, b in rep]
     idl += ["in_gfs[IDX4S(UUGF,i0,i1,i2)] = %s;" % cexpr(red[0]), "in_gfs[IDX4S(VVGF,i0,i1,i2)] = %s;" % cexpr(red[1])]
     coords = coord_functions(CoordSystem)  # registers the EigenCoord's C parameters (e.g. RMAX)
-    parts = [header(3), paramstruct_c(), coords, '#include "../oracle_runtime.h"',
+    som = sommerfeld_c(CoordSystem, 4)
+    parts = [header(3), paramstruct_c(), coords, som, '#include "../oracle_runtime.h"',
              cfunc("oracle_rhs_eval", "const paramstruct *params, double *xx[3], const double *auxevol_gfs, "
                    "const double *in_gfs, double *rhs_gfs", rhs, "interior",
                    "curvilinear scalar wave RHS (ScalarWave/ScalarWaveCurvilinear_RHSs.py:43-98)"),

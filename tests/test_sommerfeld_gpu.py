@@ -21,7 +21,15 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 TOL_EVOL = 1e-10
 CASES = [("scalarwave_cartesian", dict(xmin=-3.0, xmax=3.0, ymin=-3.0, ymax=3.0, zmin=-3.0, zmax=3.0)),
-         ("maxwell_cartesian", dict(xmin=-4.0, xmax=4.0, ymin=-4.0, ymax=4.0, zmin=-4.0, zmax=4.0, time=0.4))]
+         ("maxwell_cartesian", dict(xmin=-4.0, xmax=4.0, ymin=-4.0, ymax=4.0, zmin=-4.0, zmax=4.0, time=0.4)),
+         ("scalarwave_sinhsphericalv2", dict(AMPL=3.0)),
+         ("scalarwave_sinhcylindricalv2", dict()),
+         ("scalarwave_symtp", dict(AMAX=3.0))]
+# Curvilinear RHSs agree to 1e-12 away from the origin and axis; next to them the
+# 1/r^2 and cot(theta)/r^2 terms cancel and the device (rfm tables, fused multiply-adds)
+# and the oracle (closed forms) round differently (see test_curvilinear_gpu), and the
+# parity copies of the radiation condition carry those points into the ghost zones.
+TOL_CURVI = 1e-10
 
 
 def _setup(torch, bundle, pars, Nxx):
@@ -50,7 +58,10 @@ def test_apply_bcs_sommerfeld_matches_oracle(torch_cuda, bundle, pars):
     assert og.o.sommerfeld["fd_order"] == b.meta["sommerfeld"]["fd_order"]
     rhs = g.empty(len(b.evol))
     rhs.zero_()
-    check(b.lib.rhs_eval(g.pp(), ptr(ev.y_n), ptr(rhs), ev.s()), "rhs_eval")
+    if b.meta.get("family") == "scalarwave_curvilinear":
+        check(b.lib.rhs_eval(g.rfm, g.pp(), ptr(ev.y_n), ptr(rhs), ev.s()), "rhs_eval")
+    else:
+        check(b.lib.rhs_eval(g.pp(), ptr(ev.y_n), ptr(rhs), ev.s()), "rhs_eval")
     ev.apply_bcs_sommerfeld(ev.y_n, rhs)
     torch.cuda.synchronize()
     rhs_ref = og.rhs_eval(y)
@@ -58,7 +69,7 @@ def test_apply_bcs_sommerfeld_matches_oracle(torch_cuda, bundle, pars):
     got = rhs.cpu().numpy()
     err = rel_maxnorm_err(got, rhs_ref, floor_frac=1e-12)
     print("%s: Sommerfeld RHS at all points rel err %.2e" % (bundle, err))
-    assert err < TOL, err
+    assert err < (TOL_CURVI if "cartesian" not in bundle else TOL), err
     G = og.o.NGHOSTS
     ghost = np.ones(og.shape, bool)
     ghost[G:-G, G:-G, G:-G] = False
