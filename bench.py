@@ -239,6 +239,41 @@ def bench_batched(torch, ninst=256, nsteps=100, seed=0):
             "autotune_ms": {"x".join(map(str, k)): 1e3 * v for k, v in times.items()}}
 
 
+def bench_membound(torch, reps=10):
+    """The memory-bound fused RHS + RK4 kernels (substep 1): scalar wave at 256^3 and vacuum
+    Maxwell at 192^3, HIP-event averages. Algorithmic bytes per interior point: every
+    evolved gridfunction read once, and per output gridfunction y_n and the running total
+    read, the stage output and the running total written: 8*(NGF + 4*NGF) B."""
+    from nrpytutorial_amd.runtime import Bundle, Grid, Evolution, ptr, check
+    out = {}
+    for name, N, pars in (("scalarwave_cartesian", 256, {}),
+                          ("maxwell_cartesian", 192, dict(xmin=-4.0, xmax=4.0, ymin=-4.0, ymax=4.0, zmin=-4.0,
+                                                          zmax=4.0, time=0.4))):
+        b = Bundle(name)
+        g = Grid(b, (N, N, N), **pars)
+        ev = Evolution(g)
+        ev.initial_data()
+        ev.k_odd.copy_(ev.y_n)
+        dt = 0.5 * g.params.dxx0
+        s = ev.s()
+        f = b.lib.rhs_eval_rk4_substep
+        # substep 1: stage input k_odd, y_n and the running total read, stage output k_even
+        t = time_kernel(torch, lambda: check(f(1, g.rfm, g.pp(), ptr(ev.auxevol), ptr(ev.k_odd), ptr(ev.y_n),
+                                               ptr(ev.rt), ptr(ev.k_even), dt, s), "rhs_eval_rk4_substep"), reps)
+        ngf = len(b.evol)
+        bpp = 8 * (ngf + 4 * ngf)
+        gbs = bpp * N ** 3 / t / 1e9
+        out[name] = {"grid": [N, N, N], "avg_launch_ms": 1e3 * t, "bytes_per_point": bpp, "achieved_GBs": gbs,
+                     "hbm_frac": gbs / HBM_PEAK_GBS}
+        del ev, g
+    # reference: a 1 GiB device-to-device copy (read + write) on the same device
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    t = time_kernel(torch, lambda: dst.copy_(src), reps)
+    out["device_copy_1GiB"] = {"avg_ms": 1e3 * t, "GBs": 2 * src.numel() * 8 / t / 1e9}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,6 +284,8 @@ def main():
     ap.add_argument("--no-bl", action="store_true", help="skip the Brill-Lindquist 72x12x2 line")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint scalar-wave line")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched Brill-Lindquist sweep line")
+    ap.add_argument("--no-membound", action="store_true",
+                    help="skip the memory-bound scalar-wave 256^3 / Maxwell 192^3 kernel line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (scale for the HBM byte counters)")
     ap.add_argument("--cpu-baseline-worker", nargs=3)
@@ -354,6 +391,14 @@ def main():
                                                                batched["tile"], batched["sequential_s"],
                                                                batched["sequential_gp_per_s"], batched["speedup"],
                                                                batched["H_L2_max_rel_diff"]))
+    membound = None
+    if not a.no_membound:
+        membound = bench_membound(torch)
+        log("memory-bound fused RHS+RK4 kernels: " + ", ".join(
+            "%s %d^3 %.3f ms, %.0f GB/s algorithmic (%.0f%% of HBM)" % (k, v["grid"][0], v["avg_launch_ms"],
+                                                                       v["achieved_GBs"], 100 * v["hbm_frac"])
+            for k, v in membound.items() if "grid" in v) +
+            "; device copy %.0f GB/s" % membound["device_copy_1GiB"]["GBs"])
     adjoint = None
     if not a.no_adjoint:
         adjoint = bench_adjoint(torch)
@@ -392,7 +437,7 @@ def main():
                   "fused_substep_fp64_TFLOPs": fused_flops * npts / t_fused / 1e12,
                   "fused_flops_per_point": fused_flops, "ricci_flops_per_point": cost["Ricci_eval"]["flops"],
                   "rhs_groups": groups, "bl_72x12x2_50steps": bl, "adjoint_scalarwave_128": adjoint,
-                  "batched_bl_sweep": batched},
+                  "batched_bl_sweep": batched, "memory_bound_kernels": membound},
     }
     print(json.dumps(out))
     return 0
