@@ -1,5 +1,6 @@
 /* Scalar wave equation in a curvilinear reference metric (bundles
- * scalarwave_sinhsphericalv2, scalarwave_sinhcylindricalv2).
+ * scalarwave_sinhspherical, scalarwave_sinhsphericalv2, scalarwave_sinhcylindrical,
+ * scalarwave_sinhcylindricalv2, scalarwave_symtp, scalarwave_sinhsymtp).
  * Entry points of Tutorial-Start_to_Finish-ScalarWaveCurvilinear.ipynb's C
  * driver with rfm precompute; common declarations in nrpy_amd.h. */
 #ifndef NRPY_AMD_SCALARWAVE_CURVILINEAR_H

@@ -115,7 +115,8 @@ def build_all(names=None, oracle=True, force=False):
 
 
 ORACLE_BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2", "scalarwave_sinhsphericalv2",
-                  "scalarwave_sinhcylindricalv2", "maxwell_cartesian", "scalarwave_symtp")
+                  "scalarwave_sinhcylindricalv2", "maxwell_cartesian", "scalarwave_symtp",
+                  "scalarwave_sinhspherical", "scalarwave_sinhcylindrical", "scalarwave_sinhsymtp")
 
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("-")]

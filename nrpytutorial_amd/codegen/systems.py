@@ -37,7 +37,10 @@ from . import batch as bt
 
 CURVI_SCALARWAVE = {"scalarwave_sinhsphericalv2": "SinhSphericalv2",
                     "scalarwave_sinhcylindricalv2": "SinhCylindricalv2",
-                    "scalarwave_symtp": "SymTP"}
+                    "scalarwave_symtp": "SymTP",
+                    "scalarwave_sinhspherical": "SinhSpherical",
+                    "scalarwave_sinhcylindrical": "SinhCylindrical",
+                    "scalarwave_sinhsymtp": "SinhSymTP"}
 BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2", "maxwell_cartesian") + \
     tuple(CURVI_SCALARWAVE)
 

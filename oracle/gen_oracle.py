@@ -523,7 +523,10 @@ def bssn(CoordSystem, symmetry_axes, name):
 
 CURVI_SCALARWAVE = {"scalarwave_sinhsphericalv2": "SinhSphericalv2",
                     "scalarwave_sinhcylindricalv2": "SinhCylindricalv2",
-                    "scalarwave_symtp": "SymTP"}
+                    "scalarwave_symtp": "SymTP",
+                    "scalarwave_sinhspherical": "SinhSpherical",
+                    "scalarwave_sinhcylindrical": "SinhCylindrical",
+                    "scalarwave_sinhsymtp": "SinhSymTP"}
 
 
 if __name__ == "__main__":

@@ -111,6 +111,10 @@ def _compare_lists(h, og, G):
     ("scalarwave_cartesian", (6, 5, 4), {}),
     ("scalarwave_symtp", (12, 8, 10), {}),
     ("scalarwave_symtp", (16, 12, 4), {"AMAX": 3.0, "bScale": 0.8}),
+    ("scalarwave_sinhspherical", (16, 8, 12), {}),
+    ("scalarwave_sinhcylindrical", (12, 8, 10), {}),
+    ("scalarwave_sinhsymtp", (12, 8, 10), {}),
+    ("scalarwave_sinhsymtp", (16, 12, 4), {"AMAX": 3.0, "bScale": 0.8, "SINHWAA": 0.3}),
 ])
 def test_product_bc_lists_equal_oracle(bundle, Nxx, params):
     """Boundary struct built by the product library on the host, entry by entry
@@ -128,6 +132,9 @@ def test_product_bc_lists_equal_oracle(bundle, Nxx, params):
     ("bssn_spherical_sym2", (72, 12, 2), {"RMAX": 7.5}),
     ("scalarwave_sinhcylindricalv2", (12, 8, 10), {}),
     ("scalarwave_symtp", (12, 8, 10), {}),
+    ("scalarwave_sinhspherical", (16, 8, 12), {}),
+    ("scalarwave_sinhcylindrical", (12, 8, 10), {}),
+    ("scalarwave_sinhsymtp", (12, 8, 10), {}),
 ])
 def test_composed_map_invariants(bundle, Nxx, params):
     """Every ghost point appears once, its sources are interior points, and for

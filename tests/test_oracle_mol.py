@@ -132,3 +132,29 @@ def test_symtp_fourth_order_convergence():
     r2 = np.log(errs[1] / errs[2]) / np.log(16 / 12)
     print(errs, r1, r2)
     assert 3.4 < r1 < 4.6 and 3.4 < r2 < 4.6, (errs, r1, r2)
+
+
+SINH_CASES = [
+    ("scalarwave_sinhspherical", (8, 12, 16), dict(AMPL=3.0, SINHW=0.4)),
+    ("scalarwave_sinhcylindrical", (12, 16, 24), dict(AMPLRHO=3.0, SINHWRHO=0.4, AMPLZ=3.0, SINHWZ=0.4)),
+    ("scalarwave_sinhsymtp", (8, 12, 16), dict(AMAX=3.0, SINHWAA=0.4)),
+]
+
+
+@pytest.mark.parametrize("bundle,ns,kw", SINH_CASES, ids=[c[0] for c in SINH_CASES])
+def test_sinh_maps_fourth_order_convergence(bundle, ns, kw):
+    """SinhSpherical, SinhCylindrical and SinhSymTP (reference_metric.py sinh radial maps):
+    plane wave to t=0.5 at CFL 1. Spherical-like systems: inner half in xx0, all angles.
+    SinhCylindrical: inner half in rho and the middle half in z (the z faces are outer
+    boundaries)."""
+    errs = []
+    for n in ns:
+        if "cylindrical" in bundle:
+            Nxx, reg = (n, n, 2 * n), (slice(n // 2, 3 * n // 2), slice(None), slice(0, n // 2))
+        else:
+            Nxx, reg = (2 * n, n, 2 * n), (slice(None), slice(None), slice(0, n))
+        errs.append(_max_err(bundle, Nxx, 0.5, 1.0, reg, **kw))
+    r1 = np.log(errs[0] / errs[1]) / np.log(ns[1] / ns[0])
+    r2 = np.log(errs[1] / errs[2]) / np.log(ns[2] / ns[1])
+    print(errs, r1, r2)
+    assert 3.4 < r1 < 4.6 and 3.4 < r2 < 4.6, (errs, r1, r2)

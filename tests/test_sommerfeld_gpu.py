@@ -6,7 +6,8 @@ Maxwell with the second-order ones of the Start-to-Finish run):
   each gridfunction's max norm;
 * MoL_step_forward_in_time_sommerfeld (the drivers' Sommerfeld branch: the
   condition on the RHS, the update at every point, no ghost fill): the state
-  at every point after 10 RK4 steps to 1e-10.
+  at every point after 10 RK4 steps to 1e-10 (Cartesian) and 5e-10
+  (curvilinear).
 
 The state is the bundle's initial data plus a smooth, non-radial perturbation
 that differs per gridfunction, so the one-sided and centred radial-derivative
@@ -24,12 +25,19 @@ CASES = [("scalarwave_cartesian", dict(xmin=-3.0, xmax=3.0, ymin=-3.0, ymax=3.0,
          ("maxwell_cartesian", dict(xmin=-4.0, xmax=4.0, ymin=-4.0, ymax=4.0, zmin=-4.0, zmax=4.0, time=0.4)),
          ("scalarwave_sinhsphericalv2", dict(AMPL=3.0)),
          ("scalarwave_sinhcylindricalv2", dict()),
-         ("scalarwave_symtp", dict(AMAX=3.0))]
+         ("scalarwave_symtp", dict(AMAX=3.0)),
+         ("scalarwave_sinhspherical", dict(AMPL=3.0, SINHW=0.4)),
+         ("scalarwave_sinhcylindrical", dict(AMPLRHO=3.0, SINHWRHO=0.4, AMPLZ=3.0, SINHWZ=0.4)),
+         ("scalarwave_sinhsymtp", dict(AMAX=3.0, SINHWAA=0.4))]
 # Curvilinear RHSs agree to 1e-12 away from the origin and axis; next to them the
 # 1/r^2 and cot(theta)/r^2 terms cancel and the device (rfm tables, fused multiply-adds)
 # and the oracle (closed forms) round differently (see test_curvilinear_gpu), and the
 # parity copies of the radiation condition carry those points into the ghost zones.
 TOL_CURVI = 1e-10
+# Ten RK4 steps accumulate that near-axis rounding: 4e-11 (SinhCylindricalv2) to 1.3e-10
+# (SinhCylindrical, AMPLRHO = 3, SINHWRHO = 0.4) of the state's max norm, while a wrong
+# derivative, sign or face would show at the 1e-3 level and above.
+TOL_EVOL_CURVI = 5e-10
 
 
 def _setup(torch, bundle, pars, Nxx):
@@ -86,6 +94,8 @@ def test_mol_sommerfeld_matches_oracle(torch_cuda, bundle, pars):
     ev.step_sommerfeld(dt, 10)
     og.mol_step_sommerfeld(y, dt, 10)
     torch.cuda.synchronize()
-    err = rel_maxnorm_err(ev.y_n.cpu().numpy(), y, floor_frac=1e-12)
-    print("%s: 10 RK4 steps with Sommerfeld, rel err %.2e" % (bundle, err))
-    assert err < TOL_EVOL, err
+    got = ev.y_n.cpu().numpy()
+    err = rel_maxnorm_err(got, y, floor_frac=1e-12)
+    at = np.unravel_index(np.argmax(np.abs(got - y)), y.shape)
+    print("%s: 10 RK4 steps with Sommerfeld, rel err %.2e at (gf, i2, i1, i0) = %s" % (bundle, err, at))
+    assert err < (TOL_EVOL_CURVI if "cartesian" not in bundle else TOL_EVOL), err
