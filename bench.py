@@ -274,6 +274,36 @@ def bench_membound(torch, reps=10):
     return out
 
 
+def bench_etk(torch, ev, reps, t_ricci, t_rhs):
+    """The Einstein Toolkit plugin form (bundle bssn_etk) on the BSSN state of the main run:
+    each gridfunction its own array (the slices of that state: cctk_ash = cctk_lsh, three
+    ghost zones), one RHS evaluation = Baikal_driver_pt1_BSSN_Ricci + Baikal_driver_pt2_BSSN_RHSs,
+    against Ricci_eval + rhs_eval of the SENR layout on the same state. Also checks that
+    both forms give the same values."""
+    from nrpytutorial_amd.etk import Baikal, cctk_grid
+    g = ev.g
+    G = g.bundle.NGHOSTS
+    k = Baikal()
+    s = ev.s()
+    gh = cctk_grid((g.shape[2], g.shape[1], g.shape[0]), (G, G, G), (g.params.dxx0, g.params.dxx1, g.params.dxx2))
+    params = k.new_params(eta=g.params.eta)
+    rbar = torch.zeros((len(k.auxevol),) + g.shape, dtype=torch.float64, device="cuda")
+    rhs = torch.zeros((len(k.evol),) + g.shape, dtype=torch.float64, device="cuda")
+    evol = [ev.y_n[q] for q in range(len(k.evol))]
+    rbar_l = [rbar[q] for q in range(len(k.auxevol))]
+    rhs_l = [rhs[q] for q in range(len(k.evol))]
+    t_r = time_kernel(torch, lambda: k.ricci(gh, params, evol, rbar_l, s), reps)
+    t_h = time_kernel(torch, lambda: k.rhss(gh, params, evol, rbar_l, rhs_l, s), reps)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(g.interior(rbar), g.interior(ev.auxevol)) and
+                torch.equal(g.interior(rhs), g.interior(ev.k_even)))
+    npts = g.Nxx[0] * g.Nxx[1] * g.Nxx[2]
+    return {"grid": list(g.Nxx), "ricci_ms": 1e3 * t_r, "rhss_ms": 1e3 * t_h,
+            "senr_ricci_ms": 1e3 * t_ricci, "senr_rhs_eval_ms": 1e3 * t_rhs,
+            "rhs_evals_gp_per_s": npts / (t_r + t_h), "senr_rhs_evals_gp_per_s": npts / (t_ricci + t_rhs),
+            "equal_to_senr_kernels": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,6 +314,7 @@ def main():
     ap.add_argument("--no-bl", action="store_true", help="skip the Brill-Lindquist 72x12x2 line")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint scalar-wave line")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched Brill-Lindquist sweep line")
+    ap.add_argument("--no-etk", action="store_true", help="skip the Einstein Toolkit plugin-form line")
     ap.add_argument("--no-membound", action="store_true",
                     help="skip the memory-bound scalar-wave 256^3 / Maxwell 192^3 kernel line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -369,6 +400,13 @@ def main():
         % (1e3 * t_ricci, cost["Ricci_eval"]["flops"] * npts / t_ricci / 1e12,
            "+".join("%.3f" % (1e3 * t) for t in t_group), dom_gbs, dom_tf, 1e3 * t_rhs, 1e3 * t_bc, 1e3 * t_detg))
     traffic = measured_traffic(dom_name, N, groups[kdom])
+    etk = None
+    if not a.no_etk:
+        etk = bench_etk(torch, ev, reps, t_ricci, t_rhs)
+        log("ETK plugin form %d^3: Ricci %.3f ms + RHSs %.3f ms (SENR layout %.3f + %.3f ms), "
+            "%.3e RHS-evaluation gp/s; same values as the SENR kernels: %s"
+            % (N, etk["ricci_ms"], etk["rhss_ms"], etk["senr_ricci_ms"], etk["senr_rhs_eval_ms"],
+               etk["rhs_evals_gp_per_s"], etk["equal_to_senr_kernels"]))
     if a.calibrate_copy:
         # A plain device copy of known size, for scaling the HBM byte counters.
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -437,7 +475,7 @@ def main():
                   "fused_substep_fp64_TFLOPs": fused_flops * npts / t_fused / 1e12,
                   "fused_flops_per_point": fused_flops, "ricci_flops_per_point": cost["Ricci_eval"]["flops"],
                   "rhs_groups": groups, "bl_72x12x2_50steps": bl, "adjoint_scalarwave_128": adjoint,
-                  "batched_bl_sweep": batched, "memory_bound_kernels": membound},
+                  "batched_bl_sweep": batched, "memory_bound_kernels": membound, "etk_plugin_form": etk},
     }
     print(json.dumps(out))
     return 0

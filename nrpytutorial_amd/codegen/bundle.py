@@ -307,13 +307,15 @@ def count_body(body):
         if "=" not in line or line.startswith("/*") or line.startswith("//"):
             continue
         lhs, rhs = line.split("=", 1)
-        m = re.search(r"(\w+)\[IDX4\((\w+)GF", lhs)
+        m = re.search(r"(\w+)\[IDX4\((\w+)GF", lhs) or re.search(r"()(\w+)GF\[CCTK_GFINDEX3D\(", lhs)
         if m:
             writes.add((m.group(1), m.group(2)))
-        for a, g in re.findall(r"(\w+)\[IDX4\((\w+)GF", rhs):
+        for a, g in re.findall(r"(\w+)\[IDX4\((\w+)GF", rhs) + re.findall(r"()(\w+)GF\[CCTK_GFINDEX3D\(", rhs):
             reads.add((a, g))
         if "[IDX4(" in rhs and re.match(r"\s*\w+\[IDX4\([^)]*\)\];\s*$", rhs):
             continue  # plain load
+        if re.match(r"\s*\w+GF\[CCTK_GFINDEX3D\([^\]]*\)\];\s*$", rhs):
+            continue  # plain load, Einstein Toolkit form
         toks = _TOKEN.findall(rhs.rstrip(";"))
         prev_operand = False
         for t in toks:

@@ -39,19 +39,36 @@ MAX_LDS_BYTES = 78 * 1024
 _READ = re.compile(r"^(\s*const double \w+ = )(in_gfs|auxevol_gfs)\[IDX4\((\w+),\s*i0([+-]\d+)?,\s*i1([+-]\d+)?,"
                    r"\s*i2([+-]\d+)?\)\];$")
 _BOUNDS = re.compile(r"^\s*if \(i0 >= Nxx0 \+ NGHOSTS \|\| i1 >= Nxx1 \+ NGHOSTS \|\| i2 >= Nxx2 \+ NGHOSTS\) return;$")
+# The same reads and bounds check in the Einstein Toolkit form of a kernel: one array per
+# gridfunction, ``hDD00GF[CCTK_GFINDEX3D(cctkGH, i0-1,i1,i2)]``, interior [nrpy_lo, nrpy_hi).
+_READ_ETK = re.compile(r"^(\s*const double \w+ = )(\w+GF)\[CCTK_GFINDEX3D\(cctkGH,\s*i0([+-]\d+)?,\s*i1([+-]\d+)?,"
+                       r"\s*i2([+-]\d+)?\)\];$")
+_BOUNDS_ETK = re.compile(r"^\s*if \(i0 >= nrpy_hi0 \|\| i1 >= nrpy_hi1 \|\| i2 >= nrpy_hi2\) return;$")
+
+
+def _match(line):
+    """(statement prefix, array, gridfunction, offsets) of a stencil read, or None. In the
+    Einstein Toolkit form the array is None and the gridfunction is its pointer's name."""
+    m = _READ.match(line)
+    if m:
+        return m.group(1), m.group(2), m.group(3), tuple(int(x) if x else 0 for x in m.group(4, 5, 6))
+    m = _READ_ETK.match(line)
+    if m:
+        return m.group(1), None, m.group(2), tuple(int(x) if x else 0 for x in m.group(3, 4, 5))
+    return None
 
 
 def _reads(lines):
     """{gf: (array, set of offsets)} and first-appearance order of the stencil reads."""
     reads, order = {}, []
     for line in lines:
-        m = _READ.match(line)
+        m = _match(line)
         if m:
-            gf = m.group(3)
+            gf = m[2]
             if gf not in reads:
-                reads[gf] = (m.group(2), set())
+                reads[gf] = (m[1], set())
                 order.append(gf)
-            reads[gf][1].add(tuple(int(x) if x else 0 for x in m.group(4, 5, 6)))
+            reads[gf][1].add(m[3])
     return reads, order
 
 
@@ -72,48 +89,66 @@ def select(lines, nghosts):
     return [(g, reads[g][0]) for g in staged], H, box
 
 
-def _prologue(staged, H, box, nghosts):
+# Index-space names of the two kernel forms: where the interior starts, the extent of
+# the allocated box (bounds), the row and plane strides, and whether the box rows
+# start at an even index.
+_SENR = dict(lo=("NGHOSTS", "NGHOSTS", "NGHOSTS"),
+             n=("Nxx_plus_2NGHOSTS0", "Nxx_plus_2NGHOSTS1", "Nxx_plus_2NGHOSTS2"),
+             stride=("Nxx_plus_2NGHOSTS0", "Nxx_plus_2NGHOSTS1"),
+             even="(Nxx_plus_2NGHOSTS0 & 1) == 0")
+_ETK = dict(lo=("nrpy_lo0", "nrpy_lo1", "nrpy_lo2"),
+            n=("nrpy_lsh0", "nrpy_lsh1", "nrpy_lsh2"),
+            stride=("cctkGH.ash0", "cctkGH.ash1"),
+            even="((cctkGH.ash0 | (nrpy_lo0 - NGHOSTS)) & 1) == 0")
+
+
+def _prologue(staged, H, box, nghosts, etk=False):
     TX, TY, TZ = TILE
     BX, BY, BZ = box
     nbox = BX * BY * BZ
     nchunk = nbox // 2            # 16-byte chunks (BX is even)
     threads = TX * TY * TZ
-    arrays = sorted(set(a for _, a in staged))
+    f = _ETK if etk else _SENR
+    lo, n, st = f["lo"], f["n"], f["stride"]
+    arrays = sorted(set(g if etk else a for g, a in staged))
     aligned = " && ".join("(((size_t)%s & 15) == 0)" % a for a in arrays)
+
+    def src(g, a):
+        return g if etk else "%s + (size_t)%s * ntot_" % (a, g)
     s = []
     s.append("__shared__ __attribute__((aligned(16))) double nrpy_lds[%d][%d];" % (len(staged), nbox))
     s.append("{")
     s.append("  /* copy the (%d x %d x %d)-point box around this 16x4x4 tile of %s to LDS */"
              % (BX, BY, BZ, ", ".join(g[:-2] if g.endswith("GF") else g for g, _ in staged)))
     s.append("  const int t_ = (int)threadIdx.x + %d * ((int)threadIdx.y + %d * (int)threadIdx.z);" % (TX, TY))
-    s.append("  const int x0_ = nrpy_bx * %d, y0_ = NGHOSTS - %d + nrpy_by * %d, z0_ = NGHOSTS - %d + nrpy_bz * %d;"
-             % (TX, H, TY, H, TZ))
-    s.append("  const size_t ntot_ = (size_t)Nxx_plus_2NGHOSTS0 * Nxx_plus_2NGHOSTS1 * Nxx_plus_2NGHOSTS2;")
-    s.append("  if ((Nxx_plus_2NGHOSTS0 & 1) == 0 && %s) {" % aligned)
+    x0 = "%s - NGHOSTS + nrpy_bx * %d" % (lo[0], TX) if etk else "nrpy_bx * %d" % TX
+    s.append("  const int x0_ = %s, y0_ = %s - %d + nrpy_by * %d, z0_ = %s - %d + nrpy_bz * %d;"
+             % (x0, lo[1], H, TY, lo[2], H, TZ))
+    if not etk:
+        s.append("  const size_t ntot_ = (size_t)Nxx_plus_2NGHOSTS0 * Nxx_plus_2NGHOSTS1 * Nxx_plus_2NGHOSTS2;")
+    s.append("  if (%s && %s) {" % (f["even"], aligned))
     s.append("    const int w_ = t_ >> 6, l_ = t_ & 63;")
     s.append("    for (int it_ = 0; it_ < %d; it_++) {" % ((nchunk + threads - 1) // threads))
     s.append("      const int c_ = (it_ * %d + w_) * 64 + l_;" % (threads // 64))
     s.append("      const int row_ = c_ / %d, xc_ = c_ - row_ * %d;" % (BX // 2, BX // 2))
     s.append("      const int z_ = row_ / %d, y_ = row_ - z_ * %d;" % (BY, BY))
-    s.append("      if (c_ < %d && z0_ + z_ < Nxx_plus_2NGHOSTS2 && y0_ + y_ < Nxx_plus_2NGHOSTS1 && "
-             "x0_ + 2 * xc_ + 1 < Nxx_plus_2NGHOSTS0) {" % nchunk)
-    s.append("        const size_t off_ = ((size_t)(z0_ + z_) * Nxx_plus_2NGHOSTS1 + (y0_ + y_)) * Nxx_plus_2NGHOSTS0 + "
-             "x0_ + 2 * xc_;")
+    s.append("      if (c_ < %d && z0_ + z_ < %s && y0_ + y_ < %s && x0_ + 2 * xc_ + 1 < %s) {"
+             % (nchunk, n[2], n[1], n[0]))
+    s.append("        const size_t off_ = ((size_t)(z0_ + z_) * %s + (y0_ + y_)) * %s + x0_ + 2 * xc_;" % (st[1], st[0]))
     for q, (g, a) in enumerate(staged):
-        s.append("        __builtin_amdgcn_global_load_lds((const void *)(%s + (size_t)%s * ntot_ + off_), "
+        s.append("        __builtin_amdgcn_global_load_lds((const void *)(%s + off_), "
                  "(__attribute__((address_space(3))) void *)&nrpy_lds[%d][(it_ * %d + w_) * 128], 16, 0, 0);"
-                 % (a, g, q, threads // 64))
+                 % (src(g, a), q, threads // 64))
     s.append("      }")
     s.append("    }")
     s.append("  } else {")
     s.append("    for (int e_ = t_; e_ < %d; e_ += %d) {" % (nbox, threads))
     s.append("      const int row_ = e_ / %d, x_ = e_ - row_ * %d;" % (BX, BX))
     s.append("      const int z_ = row_ / %d, y_ = row_ - z_ * %d;" % (BY, BY))
-    s.append("      if (z0_ + z_ < Nxx_plus_2NGHOSTS2 && y0_ + y_ < Nxx_plus_2NGHOSTS1 && x0_ + x_ < Nxx_plus_2NGHOSTS0) {")
-    s.append("        const size_t off_ = ((size_t)(z0_ + z_) * Nxx_plus_2NGHOSTS1 + (y0_ + y_)) * Nxx_plus_2NGHOSTS0 + "
-             "x0_ + x_;")
+    s.append("      if (z0_ + z_ < %s && y0_ + y_ < %s && x0_ + x_ < %s) {" % (n[2], n[1], n[0]))
+    s.append("        const size_t off_ = ((size_t)(z0_ + z_) * %s + (y0_ + y_)) * %s + x0_ + x_;" % (st[1], st[0]))
     for q, (g, a) in enumerate(staged):
-        s.append("        nrpy_lds[%d][e_] = %s[(size_t)%s * ntot_ + off_];" % (q, a, g))
+        s.append("        nrpy_lds[%d][e_] = %s;" % (q, "%s[off_]" % g if etk else "%s[(size_t)%s * ntot_ + off_]" % (a, g)))
     s.append("      }")
     s.append("    }")
     s.append("  }")
@@ -133,7 +168,10 @@ def stage_kernel(kernel_src, nghosts):
     if sel is None:
         return None
     staged, H, box = sel
-    bounds = [k for k, line in enumerate(lines) if _BOUNDS.match(line)]
+    etk = staged[0][1] is None
+    if any((a is None) != etk for _, a in staged):
+        return None
+    bounds = [k for k, line in enumerate(lines) if (_BOUNDS_ETK if etk else _BOUNDS).match(line)]
     if len(bounds) != 1:
         return None
     q_of = {g: q for q, (g, _) in enumerate(staged)}
@@ -142,13 +180,13 @@ def stage_kernel(kernel_src, nghosts):
     out = []
     for k, line in enumerate(lines):
         if k == bounds[0]:
-            out += [indent + p for p in _prologue(staged, H, box, nghosts)]
+            out += [indent + p for p in _prologue(staged, H, box, nghosts, etk)]
             out.append(line)
             continue
-        m = _READ.match(line)
-        if m and m.group(3) in q_of:
-            dx, dy, dz = (int(x) if x else 0 for x in m.group(4, 5, 6))
-            out.append("%snrpy_lds[%d][nrpy_li + %d];" % (m.group(1), q_of[m.group(3)], (dz * BY + dy) * BX + dx))
+        m = _match(line)
+        if m and m[2] in q_of:
+            dx, dy, dz = m[3]
+            out.append("%snrpy_lds[%d][nrpy_li + %d];" % (m[0], q_of[m[2]], (dz * BY + dy) * BX + dx))
             continue
         out.append(line)
     return "\n".join(out)

@@ -42,7 +42,7 @@ CURVI_SCALARWAVE = {"scalarwave_sinhsphericalv2": "SinhSphericalv2",
                     "scalarwave_sinhcylindrical": "SinhCylindrical",
                     "scalarwave_sinhsymtp": "SinhSymTP"}
 BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2", "maxwell_cartesian") + \
-    tuple(CURVI_SCALARWAVE)
+    tuple(CURVI_SCALARWAVE) + ("bssn_etk",)
 
 
 def _mol_glue(has_ricci, has_detg, rhs_args_decl, rhs_call_args):
@@ -672,6 +672,9 @@ def generate(name):
         return scalarwave_curvilinear(CURVI_SCALARWAVE[name], name)
     if name == "maxwell_cartesian":
         return maxwell_cartesian()
+    if name == "bssn_etk":
+        from .etk import bssn_etk
+        return bssn_etk(name)
     raise ValueError("unknown bundle " + name)
 
 

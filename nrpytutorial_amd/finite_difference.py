@@ -171,12 +171,9 @@ def _ijk(idx):
 
 
 def _read_statement(gfname, idx, array):
-    gftype = gri.find_gftype(gfname)
-    if gftype == "AUX":
-        array = "aux_gfs"
-    elif gftype == "AUXEVOL":
-        array = "auxevol_gfs"
-    return "const double %s%s = %s[IDX4(%sGF, %s)];" % (gfname, _suffix(idx), array, gfname.upper(), _ijk(idx))
+    """``const double hDD00_i0m1_i1_i2 = <access>;`` with the access spelled by gri.gfaccess
+    (finite_difference_helpers.py:234-238), so grid::GridFuncMemAccess applies to reads too."""
+    return "const double %s%s = %s;" % (gfname, _suffix(idx), gri.gfaccess(array, gfname, _ijk(idx)))
 
 
 def FD_outputC_readblock(sympyexpr_list, upwindcontrolvec=""):

@@ -62,18 +62,28 @@ def find_gftype(varname):
 
 
 def gfaccess(gfarrayname="", varname="", ijklstring=""):
-    """Access string for a gridfunction, e.g. ``in_gfs[IDX4(HDD00GF, i0,i1,i2)]``.
+    """Access string for a gridfunction, by grid::GridFuncMemAccess (grid.py:62-111).
 
-    AUX and AUXEVOL gridfunctions always live in ``aux_gfs`` / ``auxevol_gfs``
-    (grid.py:80-85)."""
+    * ``SENRlike``: one array per group, ``in_gfs[IDX4(HDD00GF, i0,i1,i2)]``; AUX and
+      AUXEVOL gridfunctions always live in ``aux_gfs`` / ``auxevol_gfs`` (grid.py:80-85).
+    * ``ETK``: one array per gridfunction, named as the Einstein Toolkit declares it,
+      ``hDD00GF[CCTK_GFINDEX3D(cctkGH, i0,i1,i2)]``, and ``hDD00_rhsGF[...]`` when the
+      array is ``rhs_gfs`` (grid.py:92-100). Three dimensions only."""
     gftype = find_gftype(varname)
+    idx = ijklstring if ijklstring else "i0, i1, i2"
+    mode = par.parval_from_str("GridFuncMemAccess")
+    if mode == "ETK":
+        if par.parval_from_str("DIM") != 3:
+            raise ValueError("GridFuncMemAccess = ETK requires three-dimensional gridfunctions")
+        return varname + ("_rhsGF" if gfarrayname == "rhs_gfs" else "GF") + "[CCTK_GFINDEX3D(cctkGH, " + idx + ")]"
+    if mode != "SENRlike":
+        raise ValueError("grid::GridFuncMemAccess = " + mode + " not supported")
     if gftype == "AUX":
         gfarrayname = "aux_gfs"
     elif gftype == "AUXEVOL":
         gfarrayname = "auxevol_gfs"
     if not gfarrayname:
         raise ValueError("gfaccess needs an array name")
-    idx = ijklstring if ijklstring else "i0, i1, i2"
     return gfarrayname + "[IDX4(" + varname.upper() + "GF, " + idx + ")]"
 
 
