@@ -90,8 +90,10 @@ def select(lines, nghosts):
 
 
 # Index-space names of the two kernel forms: where the interior starts, the extent of
-# the allocated box (bounds), the row and plane strides, and whether the box rows
-# start at an even index.
+# the allocated box (bounds), the row and plane strides, and when the 16-byte copy
+# applies: box rows start at an even index, and the row length is even so that a
+# pair (x, x + 1) with x inside the row lies inside it entirely (the copy skips a
+# pair that crosses the row's end, so an odd row length would lose its last point).
 _SENR = dict(lo=("NGHOSTS", "NGHOSTS", "NGHOSTS"),
              n=("Nxx_plus_2NGHOSTS0", "Nxx_plus_2NGHOSTS1", "Nxx_plus_2NGHOSTS2"),
              stride=("Nxx_plus_2NGHOSTS0", "Nxx_plus_2NGHOSTS1"),
@@ -99,7 +101,7 @@ _SENR = dict(lo=("NGHOSTS", "NGHOSTS", "NGHOSTS"),
 _ETK = dict(lo=("nrpy_lo0", "nrpy_lo1", "nrpy_lo2"),
             n=("nrpy_lsh0", "nrpy_lsh1", "nrpy_lsh2"),
             stride=("cctkGH.ash0", "cctkGH.ash1"),
-            even="((cctkGH.ash0 | (nrpy_lo0 - NGHOSTS)) & 1) == 0")
+            even="((cctkGH.ash0 | nrpy_lsh0 | (nrpy_lo0 - NGHOSTS)) & 1) == 0")
 
 
 def _prologue(staged, H, box, nghosts, etk=False):

@@ -13,9 +13,10 @@ seeded flat + Gaussian state of the SENR-layout tests:
 * det(gammabar) enforcement at every point equals the SENR kernel's bit for bit.
 
 The cases cover allocated rows of even and odd length, ghost zones deeper than
-the stencils need (so the interior no longer starts at NGHOSTS), and partial
-16x4x4 tiles: both the direct global-to-LDS copy and the copy through
-registers run.
+the stencils need (so the interior no longer starts at NGHOSTS), an odd
+cctk_lsh[0] inside an even cctk_ash[0] (the last point of a row must still
+reach LDS), and partial 16x4x4 tiles: both the direct global-to-LDS copy and
+the copy through registers run.
 """
 import numpy as np
 import pytest
@@ -29,7 +30,8 @@ TOL = 1e-12
 CASES = [((14, 12, 10), (3, 3, 3), (0, 0, 0)),
          ((13, 9, 7), (3, 3, 3), (2, 1, 0)),
          ((14, 12, 10), (4, 3, 5), (2, 3, 1)),
-         ((16, 12, 10), (5, 3, 3), (0, 2, 0))]
+         ((16, 12, 10), (5, 3, 3), (0, 2, 0)),
+         ((13, 9, 7), (3, 3, 3), (1, 0, 0))]
 
 
 def _setup(torch, Nxx, ng, pad):
