@@ -20,7 +20,10 @@
  * the Python host code builds its ctypes mirror from.
  *
  * Per-system kernels (rhs_eval, Ricci_eval, ...) are declared in
- * nrpy_amd_scalarwave.h, nrpy_amd_scalarwave_curvilinear.h and nrpy_amd_bssn.h.
+ * nrpy_amd_scalarwave.h, nrpy_amd_scalarwave_curvilinear.h, nrpy_amd_maxwell.h and
+ * nrpy_amd_bssn.h. The Einstein Toolkit plugin form of the BSSN kernels (bundle
+ * bssn_etk) exports only the kernels of nrpy_amd_bssn_etk.h, none of the grid,
+ * boundary or time-stepping entry points below.
  */
 #ifndef NRPY_AMD_H
 #define NRPY_AMD_H
