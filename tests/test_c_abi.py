@@ -115,3 +115,23 @@ def test_paramstruct_layout_matches_meta():
         body = body[:body.index("} paramstruct;")]
         cnames = re.findall(r"^\s+(?:int|double) (\w+);", body, flags=re.M)
         assert cnames == names, bundle
+
+
+def test_grid_size_limits_on_host():
+    """A grid whose gridfunction exceeds 2^31 points, or whose Nxx_plus_2NGHOSTS does not
+    match Nxx + 2 NGHOSTS, is rejected with NRPY_ERR_SHAPE before any launch (so no GPU
+    is needed); the largest grid that fits passes the check."""
+    b = rt.Bundle("scalarwave_cartesian")
+    G = b.NGHOSTS
+    fake = ctypes.c_void_p(0x1000)
+
+    def call(n0, n1, n2, plus=None):
+        p = b.new_params(Nxx0=n0, Nxx1=n1, Nxx2=n2)
+        q = plus or (n0 + 2 * G, n1 + 2 * G, n2 + 2 * G)
+        p.Nxx_plus_2NGHOSTS0, p.Nxx_plus_2NGHOSTS1, p.Nxx_plus_2NGHOSTS2 = q
+        return b.lib.rhs_eval(ctypes.byref(p), fake, fake, None)
+
+    assert call(1290, 1290, 1290) == -2               # 1296^3 > 2^31 points
+    assert call(16, 16, 16, plus=(22, 22, 21)) == -2  # inconsistent ghost sizes
+    assert call(0, 16, 16) == -2
+    assert b.lib.rhs_eval(None, fake, fake, None) == -1
