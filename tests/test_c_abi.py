@@ -8,6 +8,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from nrpytutorial_amd import runtime as rt
@@ -135,3 +136,55 @@ def test_grid_size_limits_on_host():
     assert call(16, 16, 16, plus=(22, 22, 21)) == -2  # inconsistent ghost sizes
     assert call(0, 16, 16) == -2
     assert b.lib.rhs_eval(None, fake, fake, None) == -1
+
+
+C_DRIVER = r"""
+#include <stdio.h>
+#include <stdlib.h>
+#include "nrpy_amd_scalarwave_curvilinear.h"
+%s
+int main(void) {
+  paramstruct params;
+%s
+  const int Nxx[3] = {16, 8, 12};
+  double *xx[3];
+  for (int d = 0; d < 3; d++) xx[d] = (double *)malloc(sizeof(double) * (Nxx[d] + 2 * 3));
+  if (set_Nxx_dxx_invdx_params__and__xx(0, Nxx, &params, xx) != NRPY_OK) return 1;
+  double x[3];
+  xx_to_Cart(&params, xx, 5, 4, 7, x);
+  printf("%%.17g %%.17g %%.17g %%.17g\n", find_timestep(&params, xx, 0.5), x[0], x[1], x[2]);
+  if (set_Nxx_dxx_invdx_params__and__xx(2, Nxx, &params, xx) != NRPY_ERR_PARAM) return 2;
+  return 0;
+}
+"""
+
+
+def test_c_driver_links_and_matches_python(tmp_path):
+    """The reference's drivers are C: a C main() built with gcc against the headers, the
+    paramstruct written from <bundle>.json, and the bundle library runs the host-side
+    grid setup (no GPU) and agrees with the ctypes path bit for bit."""
+    bundle = "scalarwave_sinhsphericalv2"
+    meta = rt.load_meta(bundle)
+    struct = "struct __paramstruct__ {\n%s};\n" % "".join(
+        "  %s %s;\n" % ("int" if t == "int" else "double", n) for n, t, _ in meta["paramstruct"])
+    init = "".join("  params.%s = %s;\n" % (n, repr(int(d)) if t == "int" else repr(float(d)))
+                   for n, t, d in meta["paramstruct"])
+    src = tmp_path / "driver.c"
+    src.write_text(C_DRIVER % (struct, init))
+    exe = tmp_path / "driver"
+    r = subprocess.run(["gcc", "-O1", "-D__HIP_PLATFORM_AMD__", "-I", INC, "-I", "/opt/rocm/include", str(src),
+                        "-o", str(exe), "-L", rt.LIBDIR, "-lnrpy_%s" % bundle, "-L/opt/rocm/lib", "-lamdhip64",
+                        "-Wl,-rpath," + rt.LIBDIR + ":/opt/rocm/lib"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, (out.returncode, out.stderr[-2000:])
+    got = [float(v) for v in out.stdout.split()]
+    b = rt.Bundle(bundle)
+    xx = [np.zeros(n + 2 * b.NGHOSTS) for n in (16, 8, 12)]
+    p = b.new_params()
+    ptrs = (ctypes.c_void_p * 3)(*[x.ctypes.data for x in xx])
+    assert b.lib.set_Nxx_dxx_invdx_params__and__xx(0, (ctypes.c_int * 3)(16, 8, 12), ctypes.byref(p), ptrs) == 0
+    x = (ctypes.c_double * 3)()
+    b.lib.xx_to_Cart(ctypes.byref(p), ptrs, 5, 4, 7, x)
+    want = [b.lib.find_timestep(ctypes.byref(p), ptrs, 0.5)] + list(x)
+    assert got == want
