@@ -661,7 +661,23 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     return bd.write_bundle(bundle_name, parts, meta)
 
 
+# Launch shape of the memory-bound Cartesian bundles. Block-shape sweep of the fused
+# RHS + RK4 substep on MI355X: scalar wave 256^3 0.310 ms with 64x4x1 against 0.329 ms
+# with the BSSN default 64x1x4; vacuum Maxwell 192^3 0.585 against 0.616 ms.
+MEMORY_BOUND_BLOCK = {"scalarwave_cartesian": (64, 4, 1), "maxwell_cartesian": (64, 4, 1)}
+
+
 def generate(name):
+    saved = (bd.lp.BLOCK_X, bd.lp.BLOCK_Y, bd.lp.BLOCK_Z)
+    if name in MEMORY_BOUND_BLOCK:
+        bd.lp.BLOCK_X, bd.lp.BLOCK_Y, bd.lp.BLOCK_Z = MEMORY_BOUND_BLOCK[name]
+    try:
+        return _generate(name)
+    finally:
+        bd.lp.BLOCK_X, bd.lp.BLOCK_Y, bd.lp.BLOCK_Z = saved
+
+
+def _generate(name):
     if name == "scalarwave_cartesian":
         return scalarwave_cartesian()
     if name == "bssn_cartesian":
