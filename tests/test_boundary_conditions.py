@@ -10,7 +10,10 @@ CurviBoundaryConditions.py:122-185), every value set to its linear index in
 the gridfunction array, ghost zones overwritten with NaN, then one
 apply_bcs_curvilinear; each line prints "i0 i1 i2 | " and the ten values cast
 to int. The golden files are CurviBoundaryConditions/SENRout4x4x4-*.txt,
-copied unchanged into tests/golden/curvibc/.
+copied unchanged into tests/golden/curvibc/. SinhSpherical and SinhCylindrical
+have the same EigenCoord (Spherical, Cylindrical) and the same 4x4x4 index
+grid, so their boundary maps, and hence their outputs, must equal the same
+golden files.
 """
 import os
 
@@ -22,7 +25,9 @@ from nrpytutorial_amd import runtime as rt
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = {"scalarwave_sinhsphericalv2": "SENRout4x4x4-Spherical_NGHOSTS4oFD.txt",
-          "scalarwave_sinhcylindricalv2": "SENRout4x4x4-Cylindrical_NGHOSTS4oFD.txt"}
+          "scalarwave_sinhcylindricalv2": "SENRout4x4x4-Cylindrical_NGHOSTS4oFD.txt",
+          "scalarwave_sinhspherical": "SENRout4x4x4-Spherical_NGHOSTS4oFD.txt",
+          "scalarwave_sinhcylindrical": "SENRout4x4x4-Cylindrical_NGHOSTS4oFD.txt"}
 PARITY = np.array([1, 2, 3, 4, 5, 6, 7, 8, 9, 0], dtype=np.int8)
 NGF = 10
 
