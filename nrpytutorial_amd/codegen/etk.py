@@ -155,10 +155,11 @@ def etk_kernel(name, desc, body, table, interior=True):
                "".join("  " + l + "\n" for l in head + body.splitlines())))
     block = "dim3(%d, %d, %d)" % (lp.BLOCK_X, lp.BLOCK_Y, lp.BLOCK_Z)
     if interior and lp.LDS_STAGING:
-        staged = lds.stage_kernel(text, lp.LDS_STAGING)
+        tile = lp.lds_tile(name)
+        staged = lds.stage_kernel(text, lp.LDS_STAGING, tile)
         if staged is not None:
-            text, block = staged, "dim3(%d, %d, %d)" % lds.TILE
-            lp.LDS_KERNELS[name] = list(lds.TILE)
+            text, block = staged, "dim3(%d, %d, %d)" % tile
+            lp.LDS_KERNELS[name] = list(tile)
     return text, (block, [slot for _, slot in ptrs])
 
 
@@ -194,7 +195,7 @@ def entry_point(name, desc, args, fills, kernels, interior=True):
 def bssn_etk(bundle_name="bssn_etk"):
     from ..BSSN import BSSN_quantities as Bq, BSSN_RHSs as rhs, BSSN_gauge_RHSs as gaugerhs
     from ..BSSN import Enforce_Detgammabar_Constraint as EGC, BSSN_constraints as bssncon
-    from .systems import rhs_groups, ricci_groups
+    from .systems import rhs_groups, ricci_groups, lds_tile_for_group
     t0 = time.time()
     par.set_parval_from_str("grid::DIM", 3)
     par.set_parval_from_str("finite_difference::FD_CENTDERIVS_ORDER", FD_ORDER)
@@ -260,6 +261,8 @@ def bssn_etk(bundle_name="bssn_etk"):
     # BSSN right-hand sides (BaikalETK_C_kernels_codegen.py:264-304)
     kernels, flops = [], 0
     for k, grp in enumerate(rhs_groups([n for n, _ in pairs])):
+        if lds_tile_for_group(grp):
+            lp.LDS_TILES["Baikal_BSSN_RHSs_g%d" % k] = lds_tile_for_group(grp)
         sel = [e for (n, _), e in zip(pairs, evol_rhss) if n in grp]
         body = fin.FD_outputC("returnstring", sel, params="outCverbose=False", upwindcontrolvec=betaU)
         flops += bd.count_body(body)["flops"]

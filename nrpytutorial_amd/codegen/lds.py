@@ -9,11 +9,13 @@ derivatives into LDS once per tile, before any other work, so those stencil
 reads become LDS reads with immediate offsets (a fraction of the L2 latency)
 and no address arithmetic.
 
-Layout. A block is a fixed TILE = 16x4x4 tile of interior points, one lane
-per point. For every staged gridfunction the block holds the box
-(16 + 2*NGHOSTS) x (4 + 2H) x (4 + 2H) around the tile in LDS, x fastest,
+Layout. A block is a fixed tile of interior points, one lane per point:
+TILE = 16x4x4 unless the generator picks another shape for a kernel
+(loop.LDS_TILES). For every staged gridfunction the block holds the box
+(TX + 2*NGHOSTS) x (TY + 2H) x (TZ + 2H) around the tile in LDS, x fastest,
 where H is the largest stencil offset of the staged reads. The box starts
-NGHOSTS points before the tile in x, so its rows begin at an even index.
+NGHOSTS points before the tile in x, so its rows begin at an even index
+(TX is even).
 
 Loading. When every box row is 16-byte aligned in global memory (even row
 length Nxx_plus_2NGHOSTS0 and a 16-byte aligned array), each wave copies
@@ -72,7 +74,7 @@ def _reads(lines):
     return reads, order
 
 
-def select(lines, nghosts):
+def select(lines, nghosts, tile=TILE):
     """Gridfunctions to stage (those read at a point off two axes at once, i.e. with a
     mixed derivative), their y/z halo H and LDS box dimensions; None if nothing qualifies
     or the boxes would not fit."""
@@ -81,7 +83,7 @@ def select(lines, nghosts):
     if not staged:
         return None
     H = max(max(abs(x) for x in d) for g in staged for d in reads[g][1])
-    box = (TILE[0] + 2 * nghosts, TILE[1] + 2 * H, TILE[2] + 2 * H)
+    box = (tile[0] + 2 * nghosts, tile[1] + 2 * H, tile[2] + 2 * H)
     while staged and len(staged) * box[0] * box[1] * box[2] * 8 > MAX_LDS_BYTES:
         staged.pop()
     if not staged:
@@ -104,8 +106,8 @@ _ETK = dict(lo=("nrpy_lo0", "nrpy_lo1", "nrpy_lo2"),
             even="((cctkGH.ash0 | nrpy_lsh0 | (nrpy_lo0 - NGHOSTS)) & 1) == 0")
 
 
-def _prologue(staged, H, box, nghosts, etk=False):
-    TX, TY, TZ = TILE
+def _prologue(staged, H, box, nghosts, etk=False, tile=TILE):
+    TX, TY, TZ = tile
     BX, BY, BZ = box
     nbox = BX * BY * BZ
     nchunk = nbox // 2            # 16-byte chunks (BX is even)
@@ -120,8 +122,8 @@ def _prologue(staged, H, box, nghosts, etk=False):
     s = []
     s.append("__shared__ __attribute__((aligned(16))) double nrpy_lds[%d][%d];" % (len(staged), nbox))
     s.append("{")
-    s.append("  /* copy the (%d x %d x %d)-point box around this 16x4x4 tile of %s to LDS */"
-             % (BX, BY, BZ, ", ".join(g[:-2] if g.endswith("GF") else g for g, _ in staged)))
+    s.append("  /* copy the (%d x %d x %d)-point box around this %dx%dx%d tile of %s to LDS */"
+             % (BX, BY, BZ, TX, TY, TZ, ", ".join(g[:-2] if g.endswith("GF") else g for g, _ in staged)))
     s.append("  const int t_ = (int)threadIdx.x + %d * ((int)threadIdx.y + %d * (int)threadIdx.z);" % (TX, TY))
     x0 = "%s - NGHOSTS + nrpy_bx * %d" % (lo[0], TX) if etk else "nrpy_bx * %d" % TX
     s.append("  const int x0_ = %s, y0_ = %s - %d + nrpy_by * %d, z0_ = %s - %d + nrpy_bz * %d;"
@@ -161,12 +163,12 @@ def _prologue(staged, H, box, nghosts, etk=False):
     return s
 
 
-def stage_kernel(kernel_src, nghosts):
+def stage_kernel(kernel_src, nghosts, tile=TILE):
     """Kernel text with its mixed-derivative stencils staged in LDS, or None when the
     kernel has none (or no recognisable bounds check). The result must be launched
-    with blockDim = TILE."""
+    with blockDim = tile (TILE unless the caller picks another shape)."""
     lines = kernel_src.split("\n")
-    sel = select(lines, nghosts)
+    sel = select(lines, nghosts, tile)
     if sel is None:
         return None
     staged, H, box = sel
@@ -182,7 +184,7 @@ def stage_kernel(kernel_src, nghosts):
     out = []
     for k, line in enumerate(lines):
         if k == bounds[0]:
-            out += [indent + p for p in _prologue(staged, H, box, nghosts, etk)]
+            out += [indent + p for p in _prologue(staged, H, box, nghosts, etk, tile)]
             out.append(line)
             continue
         m = _match(line)

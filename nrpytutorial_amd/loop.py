@@ -44,6 +44,13 @@ BATCH_SPECS = {}
 LDS_STAGING = 0
 # kernel name -> fixed block shape of every kernel rewritten that way
 LDS_KERNELS = {}
+# kernel name -> tile shape for its staging, where a generator picks one other than lds.TILE
+LDS_TILES = {}
+
+
+def lds_tile(name):
+    from .codegen import lds
+    return tuple(LDS_TILES.get(name, lds.TILE))
 
 
 def stage_lds(name, kernel):
@@ -52,11 +59,12 @@ def stage_lds(name, kernel):
     if not LDS_STAGING:
         return kernel, None
     from .codegen import lds
-    staged = lds.stage_kernel(kernel, LDS_STAGING)
+    tile = lds_tile(name)
+    staged = lds.stage_kernel(kernel, LDS_STAGING, tile)
     if staged is None:
         return kernel, None
-    LDS_KERNELS[name] = list(lds.TILE)
-    return staged, "dim3(%d, %d, %d)" % lds.TILE
+    LDS_KERNELS[name] = list(tile)
+    return staged, "dim3(%d, %d, %d)" % tile
 
 
 def kernel_id(name):
