@@ -41,7 +41,7 @@ def _cart(torch, Nxx, seed=7):
     return b, g, ev, og, y
 
 
-# The Cartesian kernels stage mixed-derivative stencils in LDS per 16x4x4 tile. (14,12,10)
+# The Cartesian kernels stage mixed-derivative stencils in LDS per 16x4x4 (64x1x4) tile. (14,12,10)
 # has an even row length (16-byte aligned rows: direct global->LDS copies) and partial
 # tiles in x and z; (13,9,7) has an odd row length (the copy through registers) and
 # partial tiles in every direction.

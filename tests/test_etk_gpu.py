@@ -15,7 +15,7 @@ seeded flat + Gaussian state of the SENR-layout tests:
 The cases cover allocated rows of even and odd length, ghost zones deeper than
 the stencils need (so the interior no longer starts at NGHOSTS), an odd
 cctk_lsh[0] inside an even cctk_ash[0] (the last point of a row must still
-reach LDS), and partial 16x4x4 tiles: both the direct global-to-LDS copy and
+reach LDS), and partial 16x4x4 and 64x1x4 tiles: both the direct global-to-LDS copy and
 the copy through registers run.
 """
 import numpy as np
