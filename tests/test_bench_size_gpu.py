@@ -1,5 +1,6 @@
-"""BSSN at the benchmark's own size, against the C oracle: the 192^3 Cartesian grid
-and seeded flat + Gaussian state that bench.py times (seed 1).
+"""The kernels at the benchmark's own sizes, against the C oracle: BSSN on the 192^3
+Cartesian grid and seeded flat + Gaussian state that bench.py times (seed 1), and
+the scalar-wave 256^3 and Maxwell 192^3 right-hand sides of its memory-bound line.
 
 At this size the LDS-staged kernels run full 16x4x4 and 64x1x4 tiles with the
 direct global-to-LDS copy (even rows), and the grid spans every XCD's range of
@@ -65,3 +66,38 @@ def test_bssn_192_ricci_rhs_hamiltonian_match_oracle(torch_cuda):
         print("192^3 %s: worst GPU-vs-oracle %.2e (FMA-vs-no-FMA oracle %.2e there); worst ratio to its tolerance %.2f"
               % (name, e_gpu.max(), e_fma[np.argmax(e_gpu)], (e_gpu / tol).max()), flush=True)
         assert np.all(e_gpu <= tol), (name, e_gpu, e_fma)
+
+
+MEMBOUND = [("scalarwave_cartesian", 256, {}),
+            ("maxwell_cartesian", 192, dict(xmin=-4.0, xmax=4.0, ymin=-4.0, ymax=4.0, zmin=-4.0, zmax=4.0, time=0.4))]
+
+
+@pytest.mark.parametrize("bundle,N,pars", MEMBOUND, ids=[m[0] for m in MEMBOUND])
+def test_memory_bound_rhs_at_bench_size(torch_cuda, bundle, N, pars):
+    """The scalar-wave 256^3 and Maxwell 192^3 right-hand sides that bench.py times, on
+    their initial data, against the oracle: same criterion as above (1e-12, or 4x the
+    oracle's own FMA-vs-no-FMA difference where the Laplacian's cancellation makes that
+    larger)."""
+    torch = torch_cuda
+    from nrpytutorial_amd.runtime import Bundle, Grid, Evolution, ptr, check
+    from oracle_lib import Oracle, OracleGrid
+    print("case: %s %d^3 rhs_eval vs oracle" % (bundle, N), flush=True)
+    b = Bundle(bundle)
+    g = Grid(b, (N, N, N), **pars)
+    ev = Evolution(g)
+    ev.initial_data()
+    rhs = ev.k_even
+    check(b.lib.rhs_eval(g.pp(), ptr(ev.y_n), ptr(rhs), ev.s()), "rhs_eval")
+    torch.cuda.synchronize()
+    y = ev.y_n.cpu().numpy()
+    got = g.interior(rhs).cpu().numpy()
+    refs = {}
+    for fast in (False, True):
+        og = OracleGrid(Oracle(bundle, fast=fast), (N, N, N), pars)
+        refs[fast] = og.interior(og.rhs_eval(y))
+    e_gpu, e_fma = _per_gf(got, refs[False]), _per_gf(refs[True], refs[False])
+    tol = np.maximum(TOL, NOISE_FACTOR * e_fma)
+    print("%s %d^3: GPU-vs-oracle %s, FMA-vs-no-FMA oracle %s" % (bundle, N, np.array2string(e_gpu, precision=2),
+                                                                  np.array2string(e_fma, precision=2)), flush=True)
+    assert np.isfinite(got).all()
+    assert np.all(e_gpu <= tol), (e_gpu, e_fma)
