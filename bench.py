@@ -274,6 +274,42 @@ def bench_membound(torch, reps=10):
     return out
 
 
+def bench_next_rows(torch, reps=10):
+    """The rows beyond the BSSN hot path, each on one GPU:
+    * Sommerfeld outer boundary (the Maxwell Start-to-Finish choice): vacuum Maxwell 192^3 on
+      [-4,4]^3, apply_bcs_sommerfeld on the RHS (HIP-event average) and a whole RK4 step of
+      MoL_step_forward_in_time_sommerfeld (RHS + condition per substep, update at all points);
+    * curvilinear scalar wave, RK4 steps replayed from a hipGraph: SinhSphericalv2
+      128 x 64 x 128 and SymTP 128 x 64 x 128 (2-D precomputed reference-metric tables).
+    Rates are RK4 gridpoint-updates/s (4 RHS evaluations per interior point per step)."""
+    from nrpytutorial_amd.runtime import Bundle, Grid, Evolution
+    out = {}
+    b = Bundle("maxwell_cartesian")
+    N = 192
+    g = Grid(b, (N, N, N), xmin=-4.0, xmax=4.0, ymin=-4.0, ymax=4.0, zmin=-4.0, zmax=4.0, time=0.4)
+    ev = Evolution(g)
+    ev.initial_data()
+    dt = g.find_timestep(0.5)
+    t_bc = time_kernel(torch, lambda: ev.apply_bcs_sommerfeld(ev.y_n, ev.k_even), reps)
+    t_step = time_kernel(torch, lambda: ev.step_sommerfeld(dt, 1), reps)
+    out["sommerfeld_maxwell_192"] = {"apply_bcs_sommerfeld_ms": 1e3 * t_bc, "rk4_step_ms": 1e3 * t_step,
+                                     "gp_per_s": 4.0 * N ** 3 / t_step}
+    del ev, g
+    for name, Nxx, pars in (("scalarwave_sinhsphericalv2", (128, 64, 128), dict(AMPL=10.0)),
+                            ("scalarwave_symtp", (128, 64, 128), dict(AMAX=10.0))):
+        b = Bundle(name)
+        g = Grid(b, Nxx, **pars)
+        ev = Evolution(g)
+        ev.initial_data()
+        dt = g.find_timestep(0.5)
+        ev.step(dt, 2)
+        t = time_kernel(torch, lambda: ev.step(dt, 1), reps)
+        out[name] = {"grid": list(Nxx), "rk4_step_ms": 1e3 * t, "gp_per_s": 4.0 * Nxx[0] * Nxx[1] * Nxx[2] / t}
+        ev.free_graph()
+        del ev, g
+    return out
+
+
 def bench_etk(torch, ev, reps, t_ricci, t_rhs):
     """The Einstein Toolkit plugin form (bundle bssn_etk) on the BSSN state of the main run:
     each gridfunction its own array (the slices of that state: cctk_ash = cctk_lsh, three
@@ -315,6 +351,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint scalar-wave line")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched Brill-Lindquist sweep line")
     ap.add_argument("--no-etk", action="store_true", help="skip the Einstein Toolkit plugin-form line")
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the Sommerfeld Maxwell 192^3 and curvilinear scalar-wave line")
     ap.add_argument("--no-membound", action="store_true",
                     help="skip the memory-bound scalar-wave 256^3 / Maxwell 192^3 kernel line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -429,6 +467,15 @@ def main():
                                                                batched["tile"], batched["sequential_s"],
                                                                batched["sequential_gp_per_s"], batched["speedup"],
                                                                batched["H_L2_max_rel_diff"]))
+    nxt = None
+    if not a.no_next:
+        nxt = bench_next_rows(torch)
+        som = nxt["sommerfeld_maxwell_192"]
+        log("Sommerfeld Maxwell 192^3: apply_bcs_sommerfeld %.3f ms, RK4 step %.3f ms (%.3e gp/s); curvilinear "
+            "scalar wave RK4 steps: %s" % (som["apply_bcs_sommerfeld_ms"], som["rk4_step_ms"], som["gp_per_s"],
+                                          ", ".join("%s %s %.3f ms (%.3e gp/s)" % (k, "x".join(map(str, v["grid"])),
+                                                                                v["rk4_step_ms"], v["gp_per_s"])
+                                                    for k, v in nxt.items() if "grid" in v)))
     membound = None
     if not a.no_membound:
         membound = bench_membound(torch)
@@ -475,7 +522,8 @@ def main():
                   "fused_substep_fp64_TFLOPs": fused_flops * npts / t_fused / 1e12,
                   "fused_flops_per_point": fused_flops, "ricci_flops_per_point": cost["Ricci_eval"]["flops"],
                   "rhs_groups": groups, "bl_72x12x2_50steps": bl, "adjoint_scalarwave_128": adjoint,
-                  "batched_bl_sweep": batched, "memory_bound_kernels": membound, "etk_plugin_form": etk},
+                  "batched_bl_sweep": batched, "memory_bound_kernels": membound, "etk_plugin_form": etk,
+                  "sommerfeld_and_curvilinear": nxt},
     }
     print(json.dumps(out))
     return 0
