@@ -111,12 +111,15 @@ def build_all(names=None, oracle=True, force=False):
         for out in ex.map(compile_bundle, names):
             print("compiled", out, flush=True)
     if oracle:
-        build_oracle([n for n in names if n in ORACLE_BUNDLES], force=force)
+        build_oracle([n for n in ORACLE_BUNDLES if ORACLE_BUNDLES[n] in names], force=force)
 
 
-ORACLE_BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2", "scalarwave_sinhsphericalv2",
-                  "scalarwave_sinhcylindricalv2", "maxwell_cartesian", "scalarwave_symtp",
-                  "scalarwave_sinhspherical", "scalarwave_sinhcylindrical", "scalarwave_sinhsymtp")
+# oracle library -> the product bundle it checks ("_ld": the extended-precision build)
+ORACLE_BUNDLES = {n: n for n in ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2",
+                                 "scalarwave_sinhsphericalv2", "scalarwave_sinhcylindricalv2", "maxwell_cartesian",
+                                 "scalarwave_symtp", "scalarwave_sinhspherical", "scalarwave_sinhcylindrical",
+                                 "scalarwave_sinhsymtp")}
+ORACLE_BUNDLES["bssn_cartesian_ld"] = "bssn_cartesian"
 
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("-")]

@@ -185,8 +185,15 @@ def coord_host_functions(CoordSystem):
             + "".join("    " + l.replace("const double ds_dirn", "ds_dirn") + "\n" for l in ds.splitlines()) +
             "    dsmin = fmin(dsmin, fmin(ds_dirn0, fmin(ds_dirn1, ds_dirn2)));\n"
             "  }\n"
-            "  return dsmin*CFL_FACTOR/1.0;\n}\n")
+            "  return dsmin*CFL_FACTOR/%s;\n}\n" % _wavespeed_expr())
     return out
+
+
+def _wavespeed_expr():
+    """find_timestep's divisor (reference_metric.py:1434): the wavespeed C parameter where the
+    system registers one (the scalar wave), else 1.0, the value the BSSN and Maxwell
+    Start-to-Finish drivers #define."""
+    return "params->wavespeed" if any(n == "wavespeed" for n, _ in par.paramstruct_members()) else "1.0"
 
 
 def rfm_struct_code():
@@ -344,6 +351,11 @@ def write_bundle(bundle_name, parts, meta):
     directory of csrc/generated, when generating variants for tuning)."""
     gendir = os.environ.get("NRPY_GENDIR", GENDIR)
     os.makedirs(gendir, exist_ok=True)
+    from .. import finite_difference as fin
+    fdfuncs = fin.finite_difference_functions_code()
+    if fdfuncs:
+        # operator functions of FD_functions_enable go after the header (first part), before any kernel
+        parts = [parts[0], fdfuncs] + list(parts[1:])
     src = "// Generated by nrpytutorial_amd.codegen; do not edit.\n" + "\n".join(parts)
     with open(os.path.join(gendir, bundle_name + ".hip"), "w") as f:
         f.write(src)

@@ -18,11 +18,23 @@ Same interface and algorithm as finite_difference.py / finite_difference_helpers
    and the right-hand sides are CSE'd with prefix FDPart3 and written out
    (finite_difference_helpers.py:822-853).
 
+With ``finite_difference::FD_functions_enable`` set, step 5 calls one function
+per finite-difference operator instead of inlining the stencils, as
+``add_FD_func_to_outC_function_dict`` does (finite_difference_helpers.py:508-593):
+``order_<N>_f_<op>(invdx..., f_<point>...)``, registered in ``fd_function_dict``
+and written out by ``output_finite_difference_functions_h``. Only the upwind
+control vector stays inline. The functions are ``NRPY_FD_FUNC``-qualified:
+``static __device__ __forceinline__`` in the HIP bundles (a call per stencil
+would spill the ~240 live stencil values of a BSSN lane), ``static inline`` in
+plain C.
+
 The result is a per-gridpoint block of C statements that is valid HIP device
 code; ``loop.simple_loop`` / ``outputC.outCfunction`` place it in a gfx950
 kernel where one lane owns one gridpoint.
 """
+from collections import OrderedDict
 from fractions import Fraction
+import os
 import sympy as sp
 
 from . import NRPy_param_funcs as par
@@ -35,6 +47,66 @@ par.initialize_param(par.glb_param("bool", modulename, "FD_functions_enable", "F
 par.initialize_param(par.glb_param("int", modulename, "FD_KO_ORDER__CENTDERIVS_PLUS", 2))
 
 DERIV_TAGS = ("_dD", "_dupD", "_ddnD", "_dKOD")
+
+# FD_functions_enable: function name -> C definition (finite_difference_helpers.py:508-593)
+fd_function_dict = OrderedDict()
+
+FD_FUNC_QUALIFIER = """#ifndef NRPY_FD_FUNC
+#ifdef __HIPCC__
+#define NRPY_FD_FUNC static __device__ __forceinline__
+#else
+#define NRPY_FD_FUNC static inline
+#endif
+#endif
+"""
+
+
+def _fd_functions_enabled():
+    v = par.parval_from_str("finite_difference::FD_functions_enable")
+    return v is True or str(v) == "True"
+
+
+def _fd_function_call(lhs, base, op, coeffs, stencl):
+    """Register the operator function for ``op`` (once) and return the statement that calls it
+    for the gridfunction ``base`` (finite_difference_helpers.py:508-593): arguments are the
+    invdx factors the operator uses, then the stencil points in stencil order."""
+    order = par.parval_from_str("finite_difference::FD_CENTDERIVS_ORDER")
+    fname = "order_%d_f_%s" % (order, op)
+    dirs = [int(op[-2]), int(op[-1])] if op.startswith("dDD") else [int(op[-1])]
+    used = sorted(set(dirs))
+    if fname not in fd_function_dict:
+        invdx = [sp.Symbol("invdx%d" % d, real=True) for d in range(3)]
+        e = sp.sympify(0)
+        for c, st in zip(coeffs, stencl):
+            e += c * sp.Symbol("f" + _suffix(st))
+        for d in dirs:
+            e *= invdx[d]
+        body = outputC(e, "retval", "returnstring",
+                       params="includebraces=False,outCverbose=False,CSE_preprocess=True")
+        body = body.replace("retval = ", "return ")
+        args = ["const double invdx%d" % d for d in used] + ["const double f" + _suffix(st) for st in stencl]
+        desc = (op.replace("dDD", "second derivative: ").replace("dD", "first derivative: ")
+                .replace("dKOD", "Kreiss-Oliger derivative: ").replace("dupD", "upwinded derivative: ")
+                .replace("ddnD", "downwinded derivative: "))
+        fd_function_dict[fname] = ("/* Finite-difference operator, %s direction. */\n"
+                                   "NRPY_FD_FUNC double %s(%s) {\n%s}\n"
+                                   % (desc, fname, ", ".join(args), "".join("  " + l + "\n" for l in body.splitlines())))
+    callargs = ["invdx%d" % d for d in used] + [base + _suffix(st) for st in stencl]
+    return "%s = %s(%s);" % (lhs, fname, ", ".join(callargs))
+
+
+def finite_difference_functions_code():
+    """Definitions of every registered operator function (empty if none)."""
+    if not fd_function_dict:
+        return ""
+    return FD_FUNC_QUALIFIER + "".join(fd_function_dict.values())
+
+
+def output_finite_difference_functions_h(path=os.path.join(".")):
+    """Write finite_difference_functions.h (finite_difference.py:159-181)."""
+    with open(os.path.join(path, "finite_difference_functions.h"), "w") as f:
+        f.write("#ifndef __FD_FUNCTIONS_H__\n#define __FD_FUNCTIONS_H__\n#include <math.h>\n"
+                + finite_difference_functions_code() + "#endif // #ifndef __FD_FUNCTIONS_H__\n")
 
 
 def _solve_weights(offsets, m):
@@ -211,8 +283,15 @@ def FD_outputC_readblock(sympyexpr_list, upwindcontrolvec=""):
             reads.append(_read_statement(gf, s, "in_gfs"))
 
     invdx = [sp.Symbol("invdx%d" % d, real=True) for d in range(3)]
-    FDexprs, FDnames = [], []
+    FDexprs, FDnames, funccalls = [], [], []
+    use_functions = _fd_functions_enabled()
     for v, b, o, (coeffs, st) in zip(derivs, bases, ops, stencils):
+        name = str(v)
+        if upwindcontrolvec != "" and ("_dupD" in name or "_ddnD" in name):
+            name = "UpwindAlgInput" + name
+        if use_functions:
+            funccalls.append(_fd_function_call("const double " + name, b, o, coeffs, st))
+            continue
         e = sp.sympify(0)
         for c, s in zip(coeffs, st):
             e += c * sp.Symbol(b + _suffix(s))
@@ -221,9 +300,6 @@ def FD_outputC_readblock(sympyexpr_list, upwindcontrolvec=""):
         else:
             e *= invdx[int(o[-1])]
         FDexprs.append(e)
-        name = str(v)
-        if upwindcontrolvec != "" and ("_dupD" in name or "_ddnD" in name):
-            name = "UpwindAlgInput" + name
         FDnames.append("const double " + name)
 
     upwind_dirs = []
@@ -239,7 +315,13 @@ def FD_outputC_readblock(sympyexpr_list, upwindcontrolvec=""):
     if reads:
         out += "/* Step %d of %d: read gridfunctions and compute finite-difference stencils */\n" % (step, nsteps)
         step += 1
-        if FDexprs:
+        if use_functions:
+            out += "\n".join(reads + funccalls) + "\n"
+            if FDexprs:
+                out += outputC(FDexprs, FDnames, "returnstring",
+                               params="includebraces=False,CSE_varprefix=FDPart1,CSE_preprocess=True,"
+                                      "outCverbose=False")
+        elif FDexprs:
             out += outputC(FDexprs, FDnames, "returnstring",
                            params="includebraces=False,CSE_varprefix=FDPart1,CSE_preprocess=True,outCverbose=False",
                            prestring="\n".join(reads))

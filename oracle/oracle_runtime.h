@@ -141,7 +141,8 @@ void oracle_bcstruct_layer(const o_bcstruct *bc, int gz, int *outer6, int *inner
 int oracle_NGHOSTS(void) { return NGHOSTS; }
 int oracle_sizeof_paramstruct(void) { return (int)sizeof(paramstruct); }
 
-/* find_timestep (reference_metric.py:1412-1439): min over interior of scale factor * dxx. */
+/* find_timestep (reference_metric.py:1412-1439): CFL_FACTOR * min over interior of scale factor * dxx,
+ * divided by the wave speed. */
 double oracle_find_timestep(const paramstruct *params, double *xx[3], double CFL_FACTOR) {
   double dsmin = 1e38;
   for (int i2 = NGHOSTS; i2 < params->Nxx_plus_2NGHOSTS2 - NGHOSTS; i2++)
@@ -152,7 +153,7 @@ double oracle_find_timestep(const paramstruct *params, double *xx[3], double CFL
         const double d0 = s[0]*params->dxx0, d1 = s[1]*params->dxx1, d2 = s[2]*params->dxx2;
         dsmin = fmin(dsmin, fmin(d0, fmin(d1, d2)));
       }
-  return dsmin*CFL_FACTOR/1.0;
+  return dsmin*CFL_FACTOR/ORACLE_WAVESPEED(params);
 }
 
 #ifdef ORACLE_HAS_SOMMERFELD

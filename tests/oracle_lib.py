@@ -47,6 +47,11 @@ class Oracle:
             L.oracle_Hamiltonian_constraint.argtypes = [_VP, _DPP, _VP, _VP]
         if hasattr(L, "oracle_momentum_constraint"):
             L.oracle_momentum_constraint.argtypes = [_VP, _DPP, _VP, _VP]
+        # exact-derivative twins: derivative values in the order of meta["exact_derivs"][<function>]
+        if hasattr(L, "oracle_rhs_eval_exact"):
+            L.oracle_Ricci_eval_exact.argtypes = [_VP, _DPP, _VP, _VP, _VP]
+            L.oracle_rhs_eval_exact.argtypes = [_VP, _DPP, _VP, _VP, _VP, _VP]
+            L.oracle_Hamiltonian_constraint_exact.argtypes = [_VP, _DPP, _VP, _VP, _VP]
         self.sommerfeld = None
         if hasattr(L, "oracle_apply_bcs_sommerfeld"):
             L.oracle_apply_bcs_sommerfeld.argtypes = [_VP, _DPP, _VP, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP]
@@ -131,6 +136,28 @@ class OracleGrid:
         aux = self.zeros(1)
         self.o.lib.oracle_Hamiltonian_constraint(ctypes.byref(self.p), self.ptrs(self.xx), y.ctypes.data,
                                                  aux.ctypes.data)
+        return aux[0]
+
+    def exact_deriv_names(self, function):
+        """Derivative symbols the *_exact twin of ``function`` reads, in deriv_gfs order."""
+        return self.o.meta["exact_derivs"][function]
+
+    def ricci_exact(self, y, derivs):
+        aux = self.zeros(6)
+        self.o.lib.oracle_Ricci_eval_exact(ctypes.byref(self.p), self.ptrs(self.xx), y.ctypes.data,
+                                           np.ascontiguousarray(derivs).ctypes.data, aux.ctypes.data)
+        return aux
+
+    def rhs_eval_exact(self, y, auxevol, derivs):
+        out = np.zeros_like(y)
+        self.o.lib.oracle_rhs_eval_exact(ctypes.byref(self.p), self.ptrs(self.xx), auxevol.ctypes.data,
+                                         y.ctypes.data, np.ascontiguousarray(derivs).ctypes.data, out.ctypes.data)
+        return out
+
+    def hamiltonian_exact(self, y, derivs):
+        aux = self.zeros(1)
+        self.o.lib.oracle_Hamiltonian_constraint_exact(ctypes.byref(self.p), self.ptrs(self.xx), y.ctypes.data,
+                                                       np.ascontiguousarray(derivs).ctypes.data, aux.ctypes.data)
         return aux[0]
 
     def momentum(self, y):

@@ -126,7 +126,8 @@ def test_spherical_bl_initial_data_bcs(torch_cuda):
     assert err0 < TOL and err1 < TOL
 
 
-@pytest.mark.parametrize("Nxx,nsteps", [((24, 8, 2), 6), ((72, 12, 2), 3)])
+# (72, 12, 2) x 50 steps: the Brill-Lindquist head-on configuration at its stated length.
+@pytest.mark.parametrize("Nxx,nsteps", [((24, 8, 2), 6), ((72, 12, 2), 50)])
 def test_spherical_bl_evolution(torch_cuda, Nxx, nsteps):
     torch = torch_cuda
     print("case: bssn_spherical BL evolution Nxx=%s steps=%d" % (Nxx, nsteps), flush=True)

@@ -188,3 +188,23 @@ def test_c_driver_links_and_matches_python(tmp_path):
     b.lib.xx_to_Cart(ctypes.byref(p), ptrs, 5, 4, 7, x)
     want = [b.lib.find_timestep(ctypes.byref(p), ptrs, 0.5)] + list(x)
     assert got == want
+
+
+@pytest.mark.parametrize("bundle", ["scalarwave_cartesian", "scalarwave_sinhsphericalv2"])
+def test_find_timestep_divides_by_wavespeed(bundle):
+    """find_timestep returns dsmin*CFL_FACTOR/wavespeed (reference_metric.py:1434): doubling the
+    scalar wave's speed halves dt, in the bundle and in the oracle; BSSN has no wavespeed
+    parameter and divides by the drivers' 1.0."""
+    from oracle_lib import Oracle, OracleGrid
+    b = rt.Bundle(bundle)
+    Nxx = (16, 12, 10)
+    dts = {}
+    for c in (1.0, 2.0):
+        p, xx = rt.host_grid(b, Nxx, False, wavespeed=c)
+        dts[c] = b.lib.find_timestep(ctypes.byref(p), rt._xx_ptrs(xx), 0.5)
+        og = OracleGrid(Oracle(bundle), Nxx, {"wavespeed": c})
+        assert og.find_timestep(0.5) == pytest.approx(dts[c], rel=1e-15)
+    assert dts[2.0] == pytest.approx(0.5 * dts[1.0], rel=1e-15)
+    assert "wavespeed" not in [m[0] for m in rt.load_meta("bssn_cartesian")["paramstruct"]]
+    src = open(os.path.join(rt.GENDIR, "bssn_cartesian.hip")).read()
+    assert "return dsmin*CFL_FACTOR/1.0;" in src

@@ -104,3 +104,24 @@ def test_graph_replay_is_repeatable(torch_cuda):
         ev2.step(dt, 2, use_graph=False)
         torch.cuda.synchronize()
         assert torch.equal(ev.y_n, ev2.y_n), "replay %d differs from eager steps" % rep
+
+
+def test_plane_wave_64_cubed_100_steps_matches_oracle(torch_cuda):
+    """The plain scalar-wave configuration at its stated size and length: 64^3 interior on
+    [-10,10]^3, plane wave along the seed-0 direction, dt = find_timestep(0.5) = 0.5 dx,
+    100 RK4 steps (hipGraph replays), every point against the oracle's evolution."""
+    torch = torch_cuda
+    print("case: scalarwave 64^3 x 100 RK4 steps vs oracle", flush=True)
+    b, g, ev, og = _setup(torch, (64, 64, 64))
+    ev.initial_data()
+    y_ref = og.zeros(2)
+    og.initial_data(y_ref)
+    og.apply_bcs(y_ref)
+    dt = g.find_timestep(0.5)
+    assert dt == og.find_timestep(0.5) and abs(dt - 0.5 * g.params.dxx0) < 1e-15
+    ev.step(dt, nsteps=100)
+    og.mol_step(y_ref, dt, nsteps=100)
+    torch.cuda.synchronize()
+    err = rel_maxnorm_err(ev.y_n.cpu().numpy(), y_ref, report=True)
+    print("64^3 after 100 steps: rel err %.3e" % err)
+    assert err < TOL_EVOL
