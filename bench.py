@@ -310,6 +310,70 @@ def bench_next_rows(torch, reps=10):
     return out
 
 
+def bench_adjoint_bssn(torch, reps=5):
+    """Adjoint of one BSSN RK4 step (BSSNStepAdjoint.step_back: recomputation of the four
+    stages, then the transposed det(gammabar), ghost fill, RHS and Ricci) against the
+    forward step (hipGraph replay) on the same state: Brill-Lindquist data on Spherical
+    64x16x2 and the seeded flat + Gaussian state on a Cartesian 96^3 grid."""
+    from nrpytutorial_amd.runtime import Bundle, Grid, Evolution
+    from nrpytutorial_amd.adjoint import BSSNStepAdjoint
+    from nrpytutorial_amd.workloads import bssn_gaussian_state_torch
+    out = {}
+    cases = (("spherical_bl_64x16x2", "bssn_spherical_sym2", (64, 16, 2),
+              dict(BH1_mass=0.5, BH2_mass=0.5, BH1_posn_z=0.5, BH2_posn_z=-0.5, RMAX=7.5, eta=2.0)),
+             ("cartesian_96", "bssn_cartesian", (96, 96, 96), {}))
+    for label, name, Nxx, pars in cases:
+        b = Bundle(name)
+        g = Grid(b, Nxx, **pars)
+        ev = Evolution(g)
+        if name == "bssn_cartesian":
+            bssn_gaussian_state_torch(ev.y_n, g.xx_dev, b.evol, seed=1)
+            dt = 0.5 * g.params.dxx0
+        else:
+            ev.initial_data()
+            dt = g.find_timestep(0.5)
+        y0 = ev.y_n.clone()
+        adj = BSSNStepAdjoint(ev)
+        lam = torch.randn(ev.y_n.shape, dtype=torch.float64, device="cuda",
+                          generator=torch.Generator(device="cuda").manual_seed(3))
+        t_fwd = time_kernel(torch, lambda: ev.step(dt, 1), reps)
+        t_rec = time_kernel(torch, lambda: adj.recompute(y0, dt), reps)
+        t_adj = time_kernel(torch, lambda: adj.step_back(y0, lam, dt), reps)
+        out[label] = {"grid": list(Nxx), "forward_step_ms": 1e3 * t_fwd, "adjoint_step_ms": 1e3 * t_adj,
+                      "recompute_ms": 1e3 * t_rec, "adjoint_over_forward": t_adj / t_fwd}
+        ev.free_graph()
+        del adj, ev, g
+        torch.cuda.empty_cache()
+    return out
+
+
+def bench_maxwell_curvilinear(torch, nsteps=20):
+    """Vacuum Maxwell in rescaled SinhCylindrical coordinates at the sizes the reference
+    times (Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Curvilinear.ipynb:
+    50x4x100 and 80x4x160, sinh_width 0.4, CFL 0.5, toroidal dipole): RK4 steps with the
+    Sommerfeld condition (the driver's choice), and with quadratic extrapolation replayed
+    from a hipGraph. Rates are the reference's gp/s (4 RHS evaluations per point per step)."""
+    from nrpytutorial_amd.runtime import Bundle, Grid, Evolution
+    out = {}
+    b = Bundle("maxwell_sinhcylindrical")
+    for Nxx in ((50, 4, 100), (80, 4, 160)):
+        g = Grid(b, Nxx, SINHWRHO=0.4, SINHWZ=0.4)
+        ev = Evolution(g)
+        ev.initial_data(apply_bcs=False)
+        dt = g.find_timestep(0.5)
+        n = Nxx[0] * Nxx[1] * Nxx[2]
+        t_som = time_kernel(torch, lambda: ev.step_sommerfeld(dt, nsteps), 1) / nsteps
+        ev.initial_data()
+        ev.step(dt, 1)
+        t_ext = time_kernel(torch, lambda: ev.step(dt, nsteps), 1) / nsteps
+        out["x".join(map(str, Nxx))] = {"sommerfeld_step_ms": 1e3 * t_som, "sommerfeld_gp_per_s": 4.0 * n / t_som,
+                                        "extrapolation_graph_step_ms": 1e3 * t_ext,
+                                        "extrapolation_gp_per_s": 4.0 * n / t_ext}
+        ev.free_graph()
+        del ev, g
+    return out
+
+
 def bench_etk(torch, ev, reps, t_ricci, t_rhs):
     """The Einstein Toolkit plugin form (bundle bssn_etk) on the BSSN state of the main run:
     each gridfunction its own array (the slices of that state: cctk_ash = cctk_lsh, three
@@ -353,6 +417,9 @@ def main():
     ap.add_argument("--no-etk", action="store_true", help="skip the Einstein Toolkit plugin-form line")
     ap.add_argument("--no-next", action="store_true",
                     help="skip the Sommerfeld Maxwell 192^3 and curvilinear scalar-wave line")
+    ap.add_argument("--no-adjoint-bssn", action="store_true", help="skip the BSSN adjoint-step line")
+    ap.add_argument("--no-maxwell-curvi", action="store_true",
+                    help="skip the curvilinear Maxwell 50x4x100 / 80x4x160 line")
     ap.add_argument("--no-membound", action="store_true",
                     help="skip the memory-bound scalar-wave 256^3 / Maxwell 192^3 kernel line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -490,6 +557,20 @@ def main():
         log("adjoint scalar wave 128^3: forward step %.3f ms, adjoint step %.3f ms (%.2fx); 200-step loss %.3f s, "
             "gradient %.3f s" % (adjoint["forward_step_ms"], adjoint["adjoint_step_ms"], adjoint["adjoint_over_forward"],
                                  adjoint["forward_s"], adjoint["backward_s"]))
+    adjoint_bssn = None
+    if not a.no_adjoint_bssn:
+        adjoint_bssn = bench_adjoint_bssn(torch)
+        log("BSSN adjoint step: " + ", ".join(
+            "%s forward %.3f ms, adjoint %.3f ms (%.2fx; recompute %.3f ms)"
+            % (k, v["forward_step_ms"], v["adjoint_step_ms"], v["adjoint_over_forward"], v["recompute_ms"])
+            for k, v in adjoint_bssn.items()))
+    maxwell_curvi = None
+    if not a.no_maxwell_curvi:
+        maxwell_curvi = bench_maxwell_curvilinear(torch)
+        log("Maxwell SinhCylindrical: " + ", ".join(
+            "%s Sommerfeld step %.3f ms (%.3e gp/s), extrapolation graph step %.3f ms (%.3e gp/s)"
+            % (k, v["sommerfeld_step_ms"], v["sommerfeld_gp_per_s"], v["extrapolation_graph_step_ms"],
+               v["extrapolation_gp_per_s"]) for k, v in maxwell_curvi.items()))
     cpu = None
     if not a.no_cpu:
         try:
@@ -523,7 +604,8 @@ def main():
                   "fused_flops_per_point": fused_flops, "ricci_flops_per_point": cost["Ricci_eval"]["flops"],
                   "rhs_groups": groups, "bl_72x12x2_50steps": bl, "adjoint_scalarwave_128": adjoint,
                   "batched_bl_sweep": batched, "memory_bound_kernels": membound, "etk_plugin_form": etk,
-                  "sommerfeld_and_curvilinear": nxt},
+                  "sommerfeld_and_curvilinear": nxt, "adjoint_bssn": adjoint_bssn,
+                  "maxwell_sinhcylindrical": maxwell_curvi},
     }
     print(json.dumps(out))
     return 0

@@ -1,0 +1,45 @@
+/* Vacuum Maxwell System II in a curvilinear reference metric with rescaled vectors
+ * (bundle maxwell_sinhcylindrical). Entry points of
+ * Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Curvilinear.ipynb's C
+ * driver; common declarations in nrpy_amd.h. Evolved gridfunctions in alphabetical
+ * order: Gamma, aU0..2, eU0..2, psi (A^i = aU^i ReU^i, E^i = eU^i ReU^i); aux C, G. */
+#ifndef NRPY_AMD_MAXWELL_CURVILINEAR_H
+#define NRPY_AMD_MAXWELL_CURVILINEAR_H
+#include "nrpy_amd.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Replaces rhs_eval(rfm_struct *, const paramstruct *, const REAL *in_gfs, REAL *rhs_gfs) of
+ * the curvilinear Maxwell driver (VacuumMaxwell_Flat_Evol_Curvilinear_rescaled.py:11-196,
+ * System II, emitted with Enable_rfm_precompute): interior points. */
+int rhs_eval(const rfm_struct *rfmstruct, const paramstruct *params, const double *in_gfs, double *rhs_gfs,
+             hipStream_t stream);
+
+/* Replaces Constraints(rfm_struct *, const paramstruct *, REAL *in_gfs, REAL *aux_gfs) of that
+ * driver: C = Dhat_i E^i and G = Gamma - Dhat_i A^i at interior points. */
+int Maxwell_constraints(const rfm_struct *rfmstruct, const paramstruct *params, const double *in_gfs,
+                        double *aux_gfs, hipStream_t stream);
+
+/* Replaces exact_solution_all_points(const paramstruct *, REAL *xx[3], REAL *in_gfs): the
+ * toroidal dipole (Maxwell/InitialData.py:25-113) at time params->time, divided by ReU,
+ * at all points; xx are device coordinate arrays. */
+int initial_data(const paramstruct *params, double *xx[3], double *in_gfs, hipStream_t stream);
+
+/* Sommerfeld radiation condition on the RHS (CurviBoundaryConditions.py:538-620), fd_order 4,
+ * falloff power 3, speed 1, zero at infinity (the driver's settings); arguments as in
+ * nrpy_amd_maxwell.h. */
+int apply_bcs_sommerfeld(const paramstruct *params, double *xx[3], const bc_struct *bc, const int NUM_GFS,
+                         const int8_t *gfs_parity, double *gfs, double *rhs_gfs, hipStream_t stream);
+
+/* One RK4 step of the driver's BoundaryCondition = "Sommerfeld" branch: per substep rhs_eval +
+ * apply_bcs_sommerfeld, then the update at every point. */
+int MoL_step_forward_in_time_sommerfeld(const paramstruct *params, const rfm_struct *rfmstruct, double *xx[3],
+                                        const bc_struct *bcstruct, double *y_n_gfs, double *k_odd_gfs,
+                                        double *k_even_gfs, double *y_nplus1_running_total_gfs, const double dt,
+                                        hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -118,7 +118,7 @@ def build_all(names=None, oracle=True, force=False):
 ORACLE_BUNDLES = {n: n for n in ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2",
                                  "scalarwave_sinhsphericalv2", "scalarwave_sinhcylindricalv2", "maxwell_cartesian",
                                  "scalarwave_symtp", "scalarwave_sinhspherical", "scalarwave_sinhcylindrical",
-                                 "scalarwave_sinhsymtp")}
+                                 "scalarwave_sinhsymtp", "maxwell_sinhcylindrical")}
 ORACLE_BUNDLES["bssn_cartesian_ld"] = "bssn_cartesian"
 
 if __name__ == "__main__":

@@ -44,7 +44,7 @@ def batched_kernel(name, kid):
     tmpl = "template <int S>\n" if spec["template"] else ""
     kname = name + "__gpu_batched"
     kernel = ("%s__global__ void __launch_bounds__(%s) %s(%s) {\n%s%s%s}\n"
-              % (tmpl, lp.launch_bounds(), kname, ", ".join(kargs), pro, unpack,
+              % (tmpl, lp.launch_bounds(name), kname, ", ".join(kargs), pro, unpack,
                  "".join("  " + l + "\n" for l in kbody.splitlines())))
     roles = ", ".join("const int role_%s" % an for _, an in spec["arrays"])
     role_call = ", ".join("role_%s" % an for _, an in spec["arrays"])

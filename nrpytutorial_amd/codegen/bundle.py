@@ -289,7 +289,7 @@ def fused_rk4_kernel(name, args_decl, call_args, body_with_writes, loopopts="Int
             unpack += "  const int %s = params.%s;\n" % (n, n)
     kernel = ("template <int S>\n__global__ void __launch_bounds__(%s) %s__gpu(%s, double *rk_yn, double *rk_rt, "
               "double *rk_out, const double dt) {\n%s%s}\n" %
-              (lp.launch_bounds(), name, args_decl, unpack, "".join("  " + l + "\n" for l in kbody.splitlines())))
+              (lp.launch_bounds(name), name, args_decl, unpack, "".join("  " + l + "\n" for l in kbody.splitlines())))
     if "InteriorPoints" in loopopts:
         kernel = lp.stage_lds(name, kernel)[0]
     return kernel

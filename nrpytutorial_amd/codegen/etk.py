@@ -201,6 +201,8 @@ def bssn_etk(bundle_name="bssn_etk"):
     par.set_parval_from_str("finite_difference::FD_CENTDERIVS_ORDER", FD_ORDER)
     par.set_parval_from_str("reference_metric::CoordSystem", "Cartesian")
     par.set_parval_from_str("indexedexp::symmetry_axes", "")
+    from .systems import BSSN_DIFFERENCED_GFS
+    par.set_parval_from_str("finite_difference::FD_difference_from_centre", BSSN_DIFFERENCED_GFS)
     rfm.reference_metric()
     par.set_parval_from_str("BSSN.BSSN_gauge_RHSs::ShiftEvolutionOption", "GammaDriving2ndOrder_Covariant")
     par.set_parval_from_str("reference_metric::enable_rfm_precompute", "True")

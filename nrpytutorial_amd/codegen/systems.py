@@ -41,8 +41,9 @@ CURVI_SCALARWAVE = {"scalarwave_sinhsphericalv2": "SinhSphericalv2",
                     "scalarwave_sinhspherical": "SinhSpherical",
                     "scalarwave_sinhcylindrical": "SinhCylindrical",
                     "scalarwave_sinhsymtp": "SinhSymTP"}
+CURVI_MAXWELL = {"maxwell_sinhcylindrical": "SinhCylindrical"}
 BUNDLES = ("scalarwave_cartesian", "bssn_cartesian", "bssn_spherical_sym2", "maxwell_cartesian") + \
-    tuple(CURVI_SCALARWAVE) + ("bssn_etk",)
+    tuple(CURVI_SCALARWAVE) + ("bssn_etk",) + tuple(CURVI_MAXWELL)
 
 
 def _mol_glue(has_ricci, has_detg, rhs_args_decl, rhs_call_args):
@@ -477,6 +478,75 @@ def scalarwave_curvilinear(CoordSystem, bundle_name, NGHOSTS=3):
     return bd.write_bundle(bundle_name, parts, meta)
 
 
+def maxwell_curvilinear(CoordSystem, bundle_name, NGHOSTS=3):
+    """Vacuum Maxwell System II in a curvilinear reference metric with rescaled vectors, the
+    setup of Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Curvilinear.ipynb:
+    rhs_eval from VacuumMaxwellRHSs_rescaled with rfm precompute tables, toroidal-dipole data
+    A^i/ReU^i, E^i/ReU^i at time = params->time (the exact solution at that time), the
+    constraints C = Dhat_i E^i and G, the Sommerfeld condition with fd_order 4, falloff power
+    3, speed 1 and value 0 at infinity (the driver's BoundaryCondition = "Sommerfeld"), and
+    the quadratic-extrapolation step as the alternative."""
+    from ..Maxwell import InitialData as mwid, VacuumMaxwell_Flat_Evol_Curvilinear_rescaled as mw
+    par.set_parval_from_str("grid::DIM", 3)
+    par.set_parval_from_str("finite_difference::FD_CENTDERIVS_ORDER", 4)
+    par.set_parval_from_str("reference_metric::CoordSystem", CoordSystem)
+    par.set_parval_from_str("Maxwell.InitialData::System_to_use", "System_II")
+    rfm.reference_metric()
+    mwid.InitialData()
+    id_names = ["aU0", "aU1", "aU2", "eU0", "eU1", "eU2", "psi", "Gamma"]
+    id_rhss = ([mwid.AidU[i] / rfm.ReU[i] for i in range(3)] + [mwid.EidU[i] / rfm.ReU[i] for i in range(3)]
+               + [mwid.psi_ID, mwid.Gamma_ID])
+    par.set_parval_from_str("reference_metric::enable_rfm_precompute", "True")
+    rfm.reference_metric()
+    mw.VacuumMaxwellRHSs_rescaled()
+    names = ["aU0", "aU1", "aU2", "eU0", "eU1", "eU2", "psi", "Gamma"]
+    rhss = list(mw.arhsU) + list(mw.erhsU) + [mw.psi_rhs, mw.Gamma_rhs]
+    exprs = [lhrh(lhs=gri.gfaccess("rhs_gfs", n), rhs=e) for n, e in zip(names, rhss)]
+    body = fin.FD_outputC("returnstring", exprs, params="outCverbose=False")
+    rhs_src = oc.outCfunction(outfile="returnstring",
+                              desc="Evaluate the vacuum Maxwell System II RHSs, rescaled curvilinear form",
+                              name="rhs_eval",
+                              params="rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
+                                     "const REAL *restrict in_gfs,REAL *restrict rhs_gfs",
+                              body=body, loopopts="InteriorPoints,Enable_rfm_precompute")
+    gri.register_gridfunctions("AUX", ["C", "G"])
+    con_body = fin.FD_outputC("returnstring", [lhrh(lhs=gri.gfaccess("aux_gfs", "C"), rhs=mw.C),
+                                               lhrh(lhs=gri.gfaccess("aux_gfs", "G"), rhs=mw.G)],
+                              params="outCverbose=False")
+    con_src = oc.outCfunction(outfile="returnstring", desc="Evaluate the constraints C = Dhat_i E^i and G",
+                              name="Maxwell_constraints",
+                              params="rfm_struct *restrict rfmstruct,const paramstruct *restrict params,"
+                                     "const REAL *restrict in_gfs,REAL *restrict aux_gfs",
+                              body=con_body, loopopts="InteriorPoints,Enable_rfm_precompute")
+    args_decl = ("const paramstruct params, const rfm_struct rfmstruct, const double *__restrict__ auxevol_gfs, "
+                 "const double *__restrict__ in_gfs")
+    fused = bd.fused_rk4_kernel("rhs_eval_rk4_substep", args_decl, "", body,
+                                loopopts="InteriorPoints,Enable_rfm_precompute")
+    fused += _fused_launcher("rhs_eval_rk4_substep", "const rfm_struct *rfmstruct, const paramstruct *params, "
+                             "const double *auxevol_gfs, const double *in_gfs",
+                             "*params, *rfmstruct, auxevol_gfs, in_gfs", "InteriorPoints")
+    rfm_code = bd.rfm_struct_code()
+    tables = [(n, str(e), list(a)) for n, e, a in rfm.rfm_precompute_tables()]
+    par.set_parval_from_str("reference_metric::enable_rfm_precompute", "False")
+    rfm.ref_metric__hatted_quantities()
+    id_src = _initial_data_kernel(id_rhss, id_names)
+    coords = bd.coord_host_functions(CoordSystem)
+    som_src, som_meta = _sommerfeld_glue(4, "rhs_eval(rfmstruct, params, RK_INPUT_GFS, RK_OUTPUT_GFS, stream)")
+    pre, meta = _header(NGHOSTS)
+    parts = [pre, coords, '#include "../nrpy_runtime.h"', rfm_code, rhs_src, con_src, fused, id_src,
+             _mol_glue(False, False, None, "rfmstruct, params, auxevol_gfs, IN"), som_src]
+    meta["sommerfeld"] = som_meta
+    meta["cost"] = {"rhs_eval": bd.count_body(body)}
+    meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
+    meta.update({"bundle": bundle_name, "family": "maxwell_curvilinear", "CoordSystem": CoordSystem,
+                 "rfm_tables": tables, "kernels": ["rhs_eval", "Maxwell_constraints", "initial_data"],
+                 "has_ricci": False, "has_detg": False})
+    return bd.write_bundle(bundle_name, parts, meta)
+
+
+BSSN_DIFFERENCED_GFS = "alpha,cf"
+
+
 def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     from ..BSSN import BSSN_quantities as Bq, BSSN_RHSs as rhs, BSSN_gauge_RHSs as gaugerhs
     from ..BSSN import Enforce_Detgammabar_Constraint as EGC, BSSN_constraints as bssncon
@@ -486,6 +556,9 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     par.set_parval_from_str("finite_difference::FD_CENTDERIVS_ORDER", FD_order)
     par.set_parval_from_str("reference_metric::CoordSystem", CoordSystem)
     par.set_parval_from_str("indexedexp::symmetry_axes", symmetry_axes)
+    # The lapse and the conformal factor are ~1: their stencils work on differences from the
+    # centre value, so their second derivatives keep full precision (FD_difference_from_centre).
+    par.set_parval_from_str("finite_difference::FD_difference_from_centre", BSSN_DIFFERENCED_GFS)
     rfm.reference_metric()
     # Initial data is written in closed form (Read_xxs), before rfm precompute is switched on.
     bl.BrillLindquist()
@@ -533,6 +606,11 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     # curvilinear grids of the Brill-Lindquist runs (two points in phi) keep the
     # plain kernels.
     bd.lp.LDS_STAGING = (FD_order // 2 + 1) if CoordSystem == "Cartesian" else 0
+    if CoordSystem == "Cartesian":
+        # the aDD,trK group needs a few registers more than two waves per SIMD allow once its
+        # lapse and conformal-factor stencils are differenced; asking for two waves fits it
+        # (no spills) and keeps its occupancy
+        bd.lp.MIN_WAVES_PER_EU["rhs_eval_rk4_substep_g0"] = bd.lp.MIN_WAVES_PER_EU["rhs_eval_g0"] = 2
     for k, grp in enumerate(groups):
         if bd.lp.LDS_STAGING and lds_tile_for_group(grp):
             bd.lp.LDS_TILES["rhs_eval_g%d" % k] = bd.lp.LDS_TILES["rhs_eval_rk4_substep_g%d" % k] = \
@@ -704,6 +782,8 @@ def _generate(name):
         return scalarwave_curvilinear(CURVI_SCALARWAVE[name], name)
     if name == "maxwell_cartesian":
         return maxwell_cartesian()
+    if name in CURVI_MAXWELL:
+        return maxwell_curvilinear(CURVI_MAXWELL[name], name)
     if name == "bssn_etk":
         from .etk import bssn_etk
         return bssn_etk(name)

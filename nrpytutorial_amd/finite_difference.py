@@ -45,6 +45,18 @@ modulename = "finite_difference"
 par.initialize_param(par.glb_param("int", modulename, "FD_CENTDERIVS_ORDER", 4))
 par.initialize_param(par.glb_param("bool", modulename, "FD_functions_enable", "False"))
 par.initialize_param(par.glb_param("int", modulename, "FD_KO_ORDER__CENTDERIVS_PLUS", 2))
+# Gridfunctions whose centred stencils are summed as differences of nearby values (no
+# counterpart in the reference; comma-separated names, default none). The weights of a
+# centred first derivative are odd, those of a centred second derivative even and summing
+# to zero, those of a mixed second derivative odd in each direction, so
+#   dD:  sum_j w_j (f(+j) - f(-j)),
+#   dDD: sum_j w_j ((f(+j) - f(0)) + (f(-j) - f(0))),
+#   mixed dDD: sum_{a,b>0} w_a w_b ((f(a,b) - f(a,-b)) - (f(-a,b) - f(-a,-b)))
+# equal the plain weighted sums. For a field with a large constant part (the BSSN lapse
+# and conformal factor, ~1) each inner difference is exact in floating point (Sterbenz:
+# the two values within a factor 2) and small, so the O(1) part no longer leaves rounding
+# of ~1e-16/dx^2 in second derivatives. Upwinded and dissipation stencils are unchanged.
+par.initialize_param(par.glb_param("char", modulename, "FD_difference_from_centre", ""))
 
 DERIV_TAGS = ("_dD", "_dupD", "_ddnD", "_dKOD")
 
@@ -224,6 +236,62 @@ def list_deriv_vars(sympyexpr_list, upwindcontrolvec):
     return sorted(found, key=sp.default_sort_key)
 
 
+def _differenced_stencil(b, op, coeffs, stencl, stmts):
+    """The stencil of op on gridfunction b as a weighted sum of differences of nearby values
+    (FD_difference_from_centre), appending the difference statements to stmts; None for
+    stencils that are left as plain sums (upwinded, dissipation) or lack the symmetry."""
+    w = {tuple(s[:3]): c for c, s in zip(coeffs, stencl)}
+    invdx = [sp.Symbol("invdx%d" % d, real=True) for d in range(3)]
+    if op.startswith("dD") and not op.startswith("dDD") and len(op) == 3:
+        d = int(op[-1])
+        e = sp.sympify(0)
+        for off, c in w.items():
+            j = off[d]
+            if j > 0:
+                neg = tuple(-x for x in off)
+                if w.get(neg) != -c:
+                    return None
+                nm = "%s_%s_d%d" % (b, op, j)
+                stmts.append("const double %s = %s%s - %s%s;" % (nm, b, _suffix(off), b, _suffix(neg)))
+                e += c * sp.Symbol(nm)
+        return e * invdx[d]
+    if op.startswith("dDD") and op[-1] == op[-2]:
+        d = int(op[-1])
+        if sum(w.values()) != 0:
+            return None
+        e = sp.sympify(0)
+        for off, c in w.items():
+            j = off[d]
+            if j > 0:
+                neg = tuple(-x for x in off)
+                if w.get(neg) != c:
+                    return None
+                nm = "%s_%s_p%d" % (b, op, j)
+                stmts.append("const double %s = (%s%s - %s) + (%s%s - %s);"
+                             % (nm, b, _suffix(off), b, b, _suffix(neg), b))
+                e += c * sp.Symbol(nm)
+        return e * invdx[d] * invdx[d]
+    if op.startswith("dDD"):
+        d1, d2 = int(op[-2]), int(op[-1])
+        e = sp.sympify(0)
+        for off, c in w.items():
+            a, bb = off[d1], off[d2]
+            if a > 0 and bb > 0:
+                def pt(sa, sb):
+                    o = [0, 0, 0]
+                    o[d1], o[d2] = sa, sb
+                    return tuple(o)
+                if not (w.get(pt(a, -bb)) == -c and w.get(pt(-a, bb)) == -c and w.get(pt(-a, -bb)) == c):
+                    return None
+                nm = "%s_%s_q%d%d" % (b, op, a, bb)
+                stmts.append("const double %s = (%s%s - %s%s) - (%s%s - %s%s);"
+                             % (nm, b, _suffix(pt(a, bb)), b, _suffix(pt(a, -bb)),
+                                b, _suffix(pt(-a, bb)), b, _suffix(pt(-a, -bb))))
+                e += c * sp.Symbol(nm)
+        return e * invdx[d1] * invdx[d2]
+    return None
+
+
 def _suffix(idx):
     if list(idx[:3]) == [0, 0, 0]:
         return ""
@@ -266,6 +334,10 @@ def FD_outputC_readblock(sympyexpr_list, upwindcontrolvec=""):
         bases.append(b)
         ops.append(o)
     stencils = [compute_fdcoeffs_fdstencl(o) for o in ops]
+    diff_gfs = set(n.strip() for n in str(par.parval_from_str("finite_difference::FD_difference_from_centre"))
+                   .split(",") if n.strip())
+    if diff_gfs and _fd_functions_enabled():
+        raise ValueError("FD_difference_from_centre applies to inline stencils only; disable FD_functions_enable")
 
     # Points to read, per gridfunction, ordered by memory address (i2 slowest).
     points = {}
@@ -275,12 +347,21 @@ def FD_outputC_readblock(sympyexpr_list, upwindcontrolvec=""):
         for v in sp.sympify(e.rhs).free_symbols:
             if gri.variable_type(v) == "gridfunction":
                 points.setdefault(str(v), []).append((0, 0, 0, 0))
+    diffs, diff_expr = [], {}
+    for v, b, o, (coeffs, st) in zip(derivs, bases, ops, stencils):
+        if b in diff_gfs:
+            e = _differenced_stencil(b, o, coeffs, st, diffs)
+            if e is not None:
+                diff_expr[str(v)] = e
+                if o.startswith("dDD") and o[-1] == o[-2]:
+                    points[b].append((0, 0, 0, 0))  # the centre value of the pair sums
     order = {n: k for k, n in enumerate(gri.glb_gridfcs_list and [g.name for g in gri.glb_gridfcs_list])}
     reads = []
     for gf in sorted(points, key=lambda n: order.get(n, 0)):
         pts = sorted(set(points[gf]), key=lambda s: (s[2], s[1], s[0]))
         for s in pts:
             reads.append(_read_statement(gf, s, "in_gfs"))
+    reads += diffs
 
     invdx = [sp.Symbol("invdx%d" % d, real=True) for d in range(3)]
     FDexprs, FDnames, funccalls = [], [], []
@@ -291,6 +372,10 @@ def FD_outputC_readblock(sympyexpr_list, upwindcontrolvec=""):
             name = "UpwindAlgInput" + name
         if use_functions:
             funccalls.append(_fd_function_call("const double " + name, b, o, coeffs, st))
+            continue
+        if str(v) in diff_expr:
+            FDexprs.append(diff_expr[str(v)])
+            FDnames.append("const double " + name)
             continue
         e = sp.sympify(0)
         for c, s in zip(coeffs, st):

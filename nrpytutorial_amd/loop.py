@@ -27,11 +27,18 @@ BLOCK_Z = 4
 MAX_THREADS = 256
 
 
-def launch_bounds():
-    """``__launch_bounds__`` argument list. NRPY_MIN_WAVES_PER_EU (tuning) asks the
-    compiler for at least that many waves per SIMD, capping registers per lane."""
+# kernel name -> minimum waves per SIMD the compiler must fit its registers for (the
+# second __launch_bounds__ argument); set per bundle where a kernel sits just above a
+# register-occupancy step
+MIN_WAVES_PER_EU = {}
+
+
+def launch_bounds(name=None):
+    """``__launch_bounds__`` argument list. A MIN_WAVES_PER_EU entry for the kernel, or
+    NRPY_MIN_WAVES_PER_EU (tuning, every kernel), asks the compiler for at least that many
+    waves per SIMD, capping registers per lane."""
     import os
-    w = os.environ.get("NRPY_MIN_WAVES_PER_EU", "")
+    w = os.environ.get("NRPY_MIN_WAVES_PER_EU", "") or MIN_WAVES_PER_EU.get(name, "")
     return "%d, %d" % (MAX_THREADS, int(w)) if w else "%d" % MAX_THREADS
 # kernel name -> id into the runtime's tunable block-shape table
 KERNEL_IDS = {}
@@ -231,7 +238,7 @@ def kernel_and_launcher(name, desc, args, preloop, body, loopopts, postloop):
             unpack += "  const int %s = params.%s;\n" % (n, n)
     kname = name + "__gpu"
     kernel = ("/* %s */\n__global__ void __launch_bounds__(%s) %s(%s) {\n%s%s}\n" %
-              (desc, launch_bounds(), kname, ", ".join(kargs), unpack,
+              (desc, launch_bounds(name), kname, ", ".join(kargs), unpack,
                "".join("  " + l + "\n" for l in full.splitlines())))
     rng = "interior" if o["range"] == "interior" else "all"
     block = None

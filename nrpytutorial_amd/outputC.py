@@ -108,25 +108,28 @@ def _rational_name(prefix, r):
 
 
 def cse_preprocess(exprs, prefix=""):
-    """Replace non-integer rationals and integers other than 0 and +-1..+-2 by
-    named symbols so that CSE treats them as leaves; returns (exprs, map).
-    The constants print as exact ``p.0/q.0`` double divisions."""
+    """Replace non-integer rationals by named symbols so that CSE treats them as
+    leaves; returns (exprs, map). The constants print as exact ``p.0/q.0`` double
+    divisions. A rational that is the exponent of a power stays in place
+    (cse_helpers.py:67-68), so x^(1/2) and x^(1/3) still print as sqrt and cbrt
+    (``pow`` with a fractional exponent is NaN for a negative base, cbrt is not)."""
     symmap = {}
 
-    def repl(e):
-        e = sp.sympify(e)
-        rats = set()
-        for node in sp.preorder_traversal(e):
-            if isinstance(node, sp.Rational) and not isinstance(node, sp.Integer):
-                rats.add(node)
-        sub = {}
-        for r in rats:
-            name = _rational_name(prefix or "tmp", r)
-            s = sp.Symbol(name)
-            symmap[s] = r
-            sub[r] = s
-        return e.xreplace(sub) if sub else e
-    return [repl(e) for e in exprs], symmap
+    def sym(r):
+        s = sp.Symbol(_rational_name(prefix or "tmp", r))
+        symmap[s] = r
+        return s
+
+    def rebuild(e):
+        if isinstance(e, sp.Rational):
+            return e if isinstance(e, sp.Integer) else sym(e)
+        if isinstance(e, sp.Pow):
+            ex = e.exp if isinstance(e.exp, sp.Rational) else rebuild(e.exp)
+            return sp.Pow(rebuild(e.base), ex, evaluate=False)
+        if not e.args or isinstance(e, (sp.Symbol, sp.NumberSymbol)):
+            return e
+        return e.func(*[rebuild(a) for a in e.args])
+    return [rebuild(sp.sympify(e)) for e in exprs], symmap
 
 
 def outputC(sympyexpr, output_varname_str, filename="stdout", params="", prestring="", poststring=""):

@@ -90,6 +90,7 @@ KERNEL_PROTOTYPES = {
              "momentum_constraint": [_VP, _VP, _VP, _VP, _VP],
              "enforce_detgammabar_constraint": [_VP, _VP, _VP, _VP]},
     "maxwell": {"rhs_eval": [_VP, _VP, _VP, _VP], "Maxwell_constraints": [_VP, _VP, _VP, _VP]},
+    "maxwell_curvilinear": {"rhs_eval": [_VP, _VP, _VP, _VP, _VP], "Maxwell_constraints": [_VP, _VP, _VP, _VP, _VP]},
 }
 
 

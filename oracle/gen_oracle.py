@@ -494,6 +494,48 @@ def maxwell_cartesian():
                                        "evol_parity": parity_list(evol)})
 
 
+def maxwell_curvilinear(CoordSystem, name):
+    """Vacuum Maxwell System II, rescaled curvilinear form, closed-form rfm at each point
+    (Maxwell/VacuumMaxwell_Flat_Evol_Curvilinear_rescaled.py:11-196); toroidal-dipole data
+    A^i/ReU^i, E^i/ReU^i (Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Curvilinear.ipynb
+    cell 7); Sommerfeld settings of that driver (fd_order 4)."""
+    from nrpytutorial_amd.Maxwell import InitialData as mwid, VacuumMaxwell_Flat_Evol_Curvilinear_rescaled as mw
+    from nrpytutorial_amd.outputC import lhrh
+    par.set_parval_from_str("reference_metric::CoordSystem", CoordSystem)
+    par.set_parval_from_str("Maxwell.InitialData::System_to_use", "System_II")
+    rfm.reference_metric()
+    mwid.InitialData()
+    names = ["aU0", "aU1", "aU2", "eU0", "eU1", "eU2", "psi", "Gamma"]
+    idv = dict(zip(names, [mwid.AidU[i] / rfm.ReU[i] for i in range(3)] +
+                   [mwid.EidU[i] / rfm.ReU[i] for i in range(3)] + [mwid.psi_ID, mwid.Gamma_ID]))
+    mw.VacuumMaxwellRHSs_rescaled()
+    ev = dict(zip(names, list(mw.arhsU) + list(mw.erhsU) + [mw.psi_rhs, mw.Gamma_rhs]))
+    gri.register_gridfunctions("AUX", ["C", "G"])
+    rhs = point_body([lhrh("rhs_gfs[IDX4S(%sGF,i0,i1,i2)]" % n.upper(), ev[n]) for n in sorted(ev)])
+    con = point_body([lhrh("aux_gfs[IDX4S(CGF,i0,i1,i2)]", mw.C), lhrh("aux_gfs[IDX4S(GGF,i0,i1,i2)]", mw.G)])
+    rep, red = sp.cse([idv[n] for n in sorted(idv)], sp.numbered_symbols("tmp_"))
+    idl = ["const double %s = %s;" % (a, cexpr(b)) for a, b in rep]
+    idl += ["in_gfs[IDX4S(%sGF,i0,i1,i2)] = %s;" % (n.upper(), cexpr(e)) for n, e in zip(sorted(idv), red)]
+    coords = coord_functions(CoordSystem)
+    som = sommerfeld_c(CoordSystem, 4)
+    parts = [header(3), paramstruct_c(), coords, som, '#include "../oracle_runtime.h"',
+             cfunc("oracle_rhs_eval", "const paramstruct *params, double *xx[3], const double *auxevol_gfs, "
+                   "const double *in_gfs, double *rhs_gfs", rhs, "interior",
+                   "rescaled curvilinear vacuum Maxwell RHS (Maxwell/VacuumMaxwell_Flat_Evol_Curvilinear_rescaled.py)"),
+             cfunc("oracle_Maxwell_constraints", "const paramstruct *params, double *xx[3], const double *in_gfs, "
+                   "double *aux_gfs", con, "interior",
+                   "C = Dhat_i E^i and G (Maxwell/VacuumMaxwell_Flat_Evol_Curvilinear_rescaled.py)"),
+             "void oracle_Ricci_eval(const paramstruct *params, double *xx[3], const double *in_gfs, double *auxevol_gfs)"
+             " { (void)params; (void)xx; (void)in_gfs; (void)auxevol_gfs; }",
+             "void oracle_enforce_detgammabar_constraint(const paramstruct *params, double *xx[3], double *in_gfs)"
+             " { (void)params; (void)xx; (void)in_gfs; }",
+             cfunc("oracle_initial_data", "const paramstruct *params, double *xx[3], double *in_gfs", idl, "all",
+                   "toroidal dipole in the rescaled curvilinear basis (Maxwell/InitialData.py:25-113)"),
+             '#include "../oracle_mol.h"']
+    evol = gri.gridfunction_lists()[0]
+    write(name, parts, {"paramstruct": _members_with_defaults(), "evol": evol, "evol_parity": parity_list(evol)})
+
+
 def bssn(CoordSystem, symmetry_axes, name, real="double"):
     """The BSSN oracle. real="long double" builds the extended-precision variant (pointwise
     kernels only differ); the double build also gets *_exact twins of Ricci_eval, rhs_eval and
@@ -606,6 +648,8 @@ if __name__ == "__main__":
             maxwell_cartesian()
         elif b in CURVI_SCALARWAVE:
             scalarwave_curvilinear(CURVI_SCALARWAVE[b], b)
+        elif b == "maxwell_sinhcylindrical":
+            maxwell_curvilinear("SinhCylindrical", b)
         else:
             raise SystemExit("unknown oracle bundle " + b)
         print("oracle generated", b, flush=True)

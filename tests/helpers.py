@@ -54,3 +54,48 @@ def rel_maxnorm_err(a, b, floor_frac=1e-12, report=False):
         for g, d, s, e in rows:
             print("  gf %2d  max|diff| %.3e  scale %.3e  rel %.3e" % (g, d, s, e))
     return worst
+
+
+def ghost_gain(bundle, Nxx, **params):
+    """Largest sum of |coefficients| over the rows of the composed ghost map: a ghost value is
+    that linear combination of interior values, so a difference between two states' interior
+    values reaches the ghost points multiplied by at most this factor (31^3 at a corner of
+    three outermost layers of quadratic extrapolation: 10 f0 - 15 f1 + 6 f2 per direction)."""
+    from nrpytutorial_amd.runtime import HostBC
+    _, rowptr, _, coeff, _ = HostBC(bundle, Nxx, **params).composed()
+    sums = np.add.reduceat(np.abs(coeff), rowptr[:-1]) if len(coeff) else np.zeros(1)
+    return float(sums.max())
+
+
+def interior_ghost_err(a, b, G, gain, report=False):
+    """(interior error, ghost error / gain), each the per-gridfunction max|a-b| relative to
+    max|b| over interior points, maximised over gridfunctions. Two evolutions that differ by
+    rounding in the interior differ at ghost points by up to ``gain`` times that (plus the
+    fill's own rounding), so both numbers are held to the same tolerance."""
+    if not (np.isfinite(a).all() and np.isfinite(b).all()):
+        return float("inf"), float("inf")
+    inner = (slice(None), slice(G, -G), slice(G, -G), slice(G, -G))
+    mask = np.ones(a.shape[1:], bool)
+    mask[G:-G, G:-G, G:-G] = False
+    worst_i = worst_g = 0.0
+    for q in range(a.shape[0]):
+        scale = float(np.abs(b[inner][q]).max()) or 1.0
+        ei = float(np.abs(a[inner][q] - b[inner][q]).max()) / scale
+        eg = float(np.abs(a[q][mask] - b[q][mask]).max()) / scale / gain
+        if report:
+            print("  gf %2d  interior rel %.3e  ghost rel/gain %.3e  (scale %.3e)" % (q, ei, eg, scale))
+        worst_i, worst_g = max(worst_i, ei), max(worst_g, eg)
+    return worst_i, worst_g
+
+
+def per_gf_rel(a, b):
+    """max|a-b| / max|b| per gridfunction (interior arrays)."""
+    return np.array([np.abs(a[q] - b[q]).max() / max(np.abs(b[q]).max(), 1e-300) for q in range(a.shape[0])])
+
+
+def closer_to_exact(got, ref, exact, factor=2.0, floor=1e-13):
+    """The extended-precision criterion: per gridfunction, |got - exact| within ``factor``
+    times the FP64 oracle's own |ref - exact|, or below ``floor`` where both are at rounding
+    level (errors relative to each gridfunction's max |exact|). Returns (ok, e_got, e_ref)."""
+    e_got, e_ref = per_gf_rel(got, exact), per_gf_rel(ref, exact)
+    return bool(np.all(e_got <= np.maximum(factor * e_ref, floor))), e_got, e_ref

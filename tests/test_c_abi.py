@@ -21,6 +21,7 @@ FAMILY_HEADER = {"scalarwave": "nrpy_amd_scalarwave.h",
                  "scalarwave_curvilinear": "nrpy_amd_scalarwave_curvilinear.h",
                  "bssn": "nrpy_amd_bssn.h",
                  "maxwell": "nrpy_amd_maxwell.h",
+                 "maxwell_curvilinear": "nrpy_amd_maxwell_curvilinear.h",
                  "bssn_etk": "nrpy_amd_bssn_etk.h"}
 # Families whose library is only the plugin-form kernels (no grid set-up, boundary
 # conditions or time stepping from nrpy_amd.h).

@@ -21,7 +21,7 @@ the copy through registers run.
 import numpy as np
 import pytest
 
-from helpers import rel_maxnorm_err
+from helpers import rel_maxnorm_err, closer_to_exact
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -97,7 +97,11 @@ def test_plugin_form_equals_senr_kernels(torch_cuda, Nxx, ng, pad):
         assert not np.isnan(inside).any(), name
         assert np.isnan(outside).all(), name
         assert np.array_equal(inside, ref), (name, np.abs(inside - ref).max())
-    # right-hand sides against the oracle (its Ricci tensor, its RHS)
+    # right-hand sides against the oracle (its Ricci tensor, its RHS): 1e-12 of each
+    # gridfunction's max norm, or, where the oracle's own FP64 rounding is that large (second
+    # derivatives of the ~1 lapse and conformal factor, which the plugin kernels difference
+    # from the centre value), within 2x the oracle's distance to the extended-precision
+    # oracle's value
     from oracle_lib import Oracle, OracleGrid
     og = OracleGrid(Oracle("bssn_cartesian"), Nxx, {})
     ric_ref = og.ricci(y)
@@ -105,7 +109,12 @@ def test_plugin_form_equals_senr_kernels(torch_cuda, Nxx, ng, pad):
     inside, _, inner = _split(rhs_e, lsh, ng)
     err = rel_maxnorm_err(inside, rhs_ref[:, inner])
     print("rhs vs oracle %.2e" % err)
-    assert err < TOL
+    if err >= TOL:
+        ogx = OracleGrid(Oracle("bssn_cartesian_ld"), Nxx, {})
+        exact = ogx.rhs_eval(y, ogx.ricci(y))[:, inner]
+        ok, e_got, e_ref = closer_to_exact(inside, rhs_ref[:, inner], exact)
+        print("vs extended precision: plugin %.2e, oracle %.2e" % (e_got.max(), e_ref.max()))
+        assert ok, (e_got, e_ref)
     # the inputs are untouched, padding included
     for q, a in enumerate(evol):
         v = a.cpu().numpy()

@@ -114,6 +114,7 @@ import sys; sys.path.insert(0, %r)
 from nrpytutorial_amd import grid as gri, finite_difference as fin, outputC as oc, indexedexp as ixp
 from nrpytutorial_amd import NRPy_param_funcs as par
 par.set_parval_from_str("finite_difference::FD_functions_enable", %r)
+par.set_parval_from_str("finite_difference::FD_difference_from_centre", %r)
 uu, vv = gri.register_gridfunctions("EVOL", ["uu", "vv"])
 betU = ixp.register_gridfunctions_for_single_rank1("EVOL", "betU")
 uu_dD = ixp.declarerank1("uu_dD"); uu_dDD = ixp.declarerank2("uu_dDD", "sym01")
@@ -155,13 +156,19 @@ def stencil(f, axis, weights):
     return out
 
 
-@pytest.mark.parametrize("fd_functions", [False, True])
-def test_fd_outputC_end_to_end(fd_functions):
-    """Inline stencils, and with FD_functions_enable one operator function per stencil
-    (finite_difference_helpers.py:508-593): the same values either way."""
-    out = subprocess.run([sys.executable, "-c", GEN_BODY % (ROOT, fd_functions)], capture_output=True, text=True,
-                         check=True).stdout
+@pytest.mark.parametrize("fd_functions,diff", [(False, ""), (True, ""), (False, "uu,vv")])
+def test_fd_outputC_end_to_end(fd_functions, diff):
+    """Inline stencils, with FD_functions_enable one operator function per stencil
+    (finite_difference_helpers.py:508-593), and inline stencils on differences from the
+    centre value (FD_difference_from_centre): the same values every way."""
+    out = subprocess.run([sys.executable, "-c", GEN_BODY % (ROOT, fd_functions, diff)], capture_output=True,
+                         text=True, check=True).stdout
     funcs, body = out.split("//@@BODY@@")
+    if diff:
+        assert "const double uu_dDD12_q11 = (uu_i0_i1p1_i2p1 - uu_i0_i1p1_i2m1) - (uu_i0_i1m1_i2p1 - uu_i0_i1m1_i2m1);" \
+            in body
+        assert "const double vv_dDD00_p2 = (vv_i0p2_i1_i2 - vv) + (vv_i0m2_i1_i2 - vv);" in body
+        assert "const double uu_dD0_d1 = uu_i0p1_i1_i2 - uu_i0m1_i1_i2;" in body
     if fd_functions:
         assert "order_4_f_dDD12(invdx1, invdx2," in body and "order_4_f_dKOD1(" in body
         assert "NRPY_FD_FUNC double order_4_f_dupD2(" in funcs

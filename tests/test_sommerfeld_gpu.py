@@ -28,7 +28,8 @@ CASES = [("scalarwave_cartesian", dict(xmin=-3.0, xmax=3.0, ymin=-3.0, ymax=3.0,
          ("scalarwave_symtp", dict(AMAX=3.0)),
          ("scalarwave_sinhspherical", dict(AMPL=3.0, SINHW=0.4)),
          ("scalarwave_sinhcylindrical", dict(AMPLRHO=3.0, SINHWRHO=0.4, AMPLZ=3.0, SINHWZ=0.4)),
-         ("scalarwave_sinhsymtp", dict(AMAX=3.0, SINHWAA=0.4))]
+         ("scalarwave_sinhsymtp", dict(AMAX=3.0, SINHWAA=0.4)),
+         ("maxwell_sinhcylindrical", dict(AMPLRHO=3.0, SINHWRHO=0.4, AMPLZ=3.0, SINHWZ=0.4, time=0.4))]
 # Curvilinear RHSs agree to 1e-12 away from the origin and axis; next to them the
 # 1/r^2 and cot(theta)/r^2 terms cancel and the device (rfm tables, fused multiply-adds)
 # and the oracle (closed forms) round differently (see test_curvilinear_gpu), and the
@@ -66,7 +67,7 @@ def test_apply_bcs_sommerfeld_matches_oracle(torch_cuda, bundle, pars):
     assert og.o.sommerfeld["fd_order"] == b.meta["sommerfeld"]["fd_order"]
     rhs = g.empty(len(b.evol))
     rhs.zero_()
-    if b.meta.get("family") == "scalarwave_curvilinear":
+    if b.meta.get("family") in ("scalarwave_curvilinear", "maxwell_curvilinear"):
         check(b.lib.rhs_eval(g.rfm, g.pp(), ptr(ev.y_n), ptr(rhs), ev.s()), "rhs_eval")
     else:
         check(b.lib.rhs_eval(g.pp(), ptr(ev.y_n), ptr(rhs), ev.s()), "rhs_eval")

@@ -12,14 +12,17 @@ Ricci, RHS, RK4 update loop, the layer-by-layer ghost fill, det(gammabar).
   6 steps from the seeded flat + Gaussian state.
 * 192^3 with seed 1 and dt = 0.5 dx: the benchmark's own state and step, 1 step.
 
-State after the steps: 1e-10 of each gridfunction's max norm at every point (the
-evolution tolerance of the other GPU tests), and the Hamiltonian-constraint L2 norm
-within 1e-10 relative.
+State after the steps: 1e-10 of each gridfunction's interior max norm at interior
+points (the evolution tolerance of the other GPU tests), the same at ghost points
+after dividing by the composed ghost map's largest coefficient sum (a corner ghost
+value is 10 f0 - 15 f1 + 6 f2 in each of three directions, so rounding differences
+of the interior reach it multiplied by up to 31^3), and the Hamiltonian-constraint
+L2 norm within 1e-10 relative.
 """
 import numpy as np
 import pytest
 
-from helpers import rel_maxnorm_err
+from helpers import ghost_gain, interior_ghost_err
 
 pytestmark = pytest.mark.gpu
 TOL_EVOL = 1e-10
@@ -46,23 +49,24 @@ def _run(torch, Nxx, seed, nsteps):
     og.mol_step(y, dt, nsteps)
     H_ref = og.interior(og.hamiltonian(y))
     l2_ref = float(np.sqrt(np.mean(H_ref ** 2)))
-    return got, y, l2, l2_ref
+    return got, y, l2, l2_ref, ghost_gain(b, Nxx)
 
 
 @pytest.mark.parametrize("Nxx", [(14, 12, 10), (13, 9, 7)])
 def test_graph_rk4_steps_match_oracle(torch_cuda, Nxx):
     print("case: bssn_cartesian MoL_graph 6 RK4 steps vs oracle, Nxx=%s, seed 7" % (Nxx,), flush=True)
-    got, ref, l2, l2_ref = _run(torch_cuda, Nxx, 7, 6)
-    err = rel_maxnorm_err(got, ref, report=True)
-    print("state %.2e, H-L2 gpu %.15e oracle %.15e" % (err, l2, l2_ref))
-    assert err < TOL_EVOL
+    got, ref, l2, l2_ref, gain = _run(torch_cuda, Nxx, 7, 6)
+    ei, eg = interior_ghost_err(got, ref, 3, gain, report=True)
+    print("interior %.2e, ghosts/%.0f %.2e, H-L2 gpu %.15e oracle %.15e" % (ei, gain, eg, l2, l2_ref))
+    assert ei < TOL_EVOL and eg < TOL_EVOL
     assert abs(l2 - l2_ref) <= TOL_EVOL * l2_ref
 
 
 def test_graph_rk4_step_192_matches_oracle(torch_cuda):
     print("case: bssn_cartesian MoL_graph 1 RK4 step vs oracle, 192^3, seed 1", flush=True)
-    got, ref, l2, l2_ref = _run(torch_cuda, (192, 192, 192), 1, 1)
-    err = rel_maxnorm_err(got, ref, report=True)
-    print("192^3 state %.2e, H-L2 gpu %.15e oracle %.15e rel %.2e" % (err, l2, l2_ref, abs(l2 - l2_ref) / l2_ref))
-    assert err < TOL_EVOL
+    got, ref, l2, l2_ref, gain = _run(torch_cuda, (192, 192, 192), 1, 1)
+    ei, eg = interior_ghost_err(got, ref, 3, gain, report=True)
+    print("192^3 interior %.2e, ghosts/%.0f %.2e, H-L2 gpu %.15e oracle %.15e rel %.2e"
+          % (ei, gain, eg, l2, l2_ref, abs(l2 - l2_ref) / l2_ref))
+    assert ei < TOL_EVOL and eg < TOL_EVOL
     assert abs(l2 - l2_ref) <= TOL_EVOL * l2_ref
