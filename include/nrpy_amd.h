@@ -181,6 +181,16 @@ int nrpy_adjoint_combine(size_t n, double *acc, const double *L, const double *w
 int gridfunction_sumsq(const paramstruct *params, const double *gfs, int gf, double *workspace, double *result,
                        hipStream_t stream);
 
+/* gridfunction_sumsq without the workspace argument, allocating nothing (no counterpart
+ * in the reference): for callers that cannot allocate device memory, such as code under
+ * hipGraph stream capture or a serving loop that forbids hipMalloc. The partial sums live
+ * in a fixed 8 KiB device array (1024 doubles, enough for any grid: the reduction
+ * strides over the points) that the library keeps on each GPU it runs on until the
+ * process ends. Calls that overlap in time on one GPU share that array and race: order
+ * them. Same result as gridfunction_sumsq, bit for bit. */
+int gridfunction_sumsq_noalloc(const paramstruct *params, const double *gfs, int gf, double *result,
+                               hipStream_t stream);
+
 /* Block shape (threads per block <= 256, a multiple of 64) for the kernel with
  * id k (ids per bundle in <bundle>.json "kernel_ids"); (0,0,0) restores the
  * default. No counterpart in the reference. */

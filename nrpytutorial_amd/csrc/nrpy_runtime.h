@@ -28,6 +28,7 @@
 #include <vector>
 #include <map>
 #include <algorithm>
+#include <mutex>
 
 // ---------------------------------------------------------------------------
 // Data structures of BCs_data_structs.h:2-49.
@@ -788,6 +789,58 @@ extern "C" int gridfunction_sumsq(const paramstruct *params, const double *gfs, 
   hipLaunchKernelGGL(nrpy_sumsq_partial_kernel, dim3(nb), dim3(NRPY_RED_BLOCK), 0, stream, *params,
                      gfs + (size_t)gf * Ntot, workspace);
   hipLaunchKernelGGL(nrpy_sum_final_kernel, dim3(1), dim3(NRPY_RED_BLOCK), 0, stream, nb, workspace, result_dev);
+  return hipGetLastError() == hipSuccess ? NRPY_OK : NRPY_ERR_LAUNCH;
+}
+
+// The same sum with no workspace argument and no allocation, for callers that
+// cannot allocate device memory (code under stream capture, serving loops): the
+// per-block partial sums live in 'scratch', a file-scope device array of
+// NRPY_RED_MAXBLOCKS doubles (8 KiB) that each GPU running this library holds
+// until the process ends. The partial kernel writes every entry the final kernel
+// reads, so nothing needs clearing between calls. Calls on one GPU share the
+// array: calls that overlap in time on that GPU (different streams, or host
+// threads) race and must be ordered by the caller.
+static __device__ __align__(256) double scratch[NRPY_RED_MAXBLOCKS];
+
+#define NRPY_MAX_DEVICES 64
+static double *nrpy_scratch_dev[NRPY_MAX_DEVICES];  // device address of 'scratch', per device
+static std::mutex nrpy_scratch_lock;
+
+// Device address of the current device's copy of 'scratch', looked up once per
+// device with that device current (the host-side name is only a stand-in).
+static int nrpy_scratch_address(double **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return NRPY_ERR_LAUNCH;
+  if (dev < 0 || dev >= NRPY_MAX_DEVICES) return NRPY_ERR_TOO_LARGE;
+  std::lock_guard<std::mutex> guard(nrpy_scratch_lock);
+  if (!nrpy_scratch_dev[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return NRPY_ERR_LAUNCH;
+    nrpy_scratch_dev[dev] = (double *)p;
+  }
+  *out = nrpy_scratch_dev[dev];
+  return NRPY_OK;
+}
+
+extern "C" int gridfunction_sumsq_noalloc(const paramstruct *params, const double *gfs, int gf, double *result_dev,
+                                          hipStream_t stream) {
+  if (!params || !gfs || !result_dev) return NRPY_ERR_NULL;
+  if (nrpy_check_params(params)) return NRPY_ERR_SHAPE;
+  if (gf < 0) return NRPY_ERR_PARAM;
+  const size_t Ntot =
+      (size_t)params->Nxx_plus_2NGHOSTS0 * params->Nxx_plus_2NGHOSTS1 * params->Nxx_plus_2NGHOSTS2;
+  const long n = (long)params->Nxx0 * params->Nxx1 * params->Nxx2;
+  // one partial per block; the grid-stride loop bounds the block count for any grid
+  const long want = (n + NRPY_RED_BLOCK - 1) / NRPY_RED_BLOCK;
+  const int nb = (int)std::max<long>(1, std::min<long>(want, NRPY_RED_MAXBLOCKS));
+  if (nb > NRPY_RED_MAXBLOCKS) return NRPY_ERR_TOO_LARGE;
+  double *ptr = nullptr;
+  const int rc = nrpy_scratch_address(&ptr);
+  if (rc != NRPY_OK) return rc;
+  hipLaunchKernelGGL(nrpy_sumsq_partial_kernel, dim3(nb), dim3(NRPY_RED_BLOCK), 0, stream, *params,
+                     gfs + (size_t)gf * Ntot, ptr);
+  hipLaunchKernelGGL(nrpy_sum_final_kernel, dim3(1), dim3(NRPY_RED_BLOCK), 0, stream, nb, (const double *)ptr,
+                     result_dev);
   return hipGetLastError() == hipSuccess ? NRPY_OK : NRPY_ERR_LAUNCH;
 }
 

@@ -64,6 +64,7 @@ PROTOTYPES = {
     "apply_bcs_curvilinear": (_I, [_VP, _VP, _I, _VP, _VP, _VP]),
     "apply_bcs_curvilinear_composed": (_I, [_VP, _VP, _I, _VP, _VP, _VP]),
     "gridfunction_sumsq": (_I, [_VP, _VP, _I, _VP, _VP, _VP]),
+    "gridfunction_sumsq_noalloc": (_I, [_VP, _VP, _I, _VP, _VP]),
     "rfm_struct_create": (_I, [_VP, _DPP, ctypes.POINTER(_VP)]),
     "rfm_struct_free": (None, [_VP]),
     "rhs_eval_rk4_substep": (_I, [_I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _D, _VP]),
