@@ -56,6 +56,12 @@ int MoL_step_forward_in_time_sommerfeld(const paramstruct *params, const rfm_str
                                         double *k_even_gfs, double *y_nplus1_running_total_gfs, const double dt,
                                         hipStream_t stream);
 
+/* The Sommerfeld step above captured into a hipGraph (no counterpart in the reference; pointers, xx
+ * and dt fixed at capture), replayed nsteps times per MoL_graph_launch, freed with MoL_graph_free. */
+int MoL_sommerfeld_graph_create(const paramstruct *params, const rfm_struct *rfmstruct, double *xx[3],
+                                const bc_struct *bcstruct, double *y_n_gfs, double *k_odd_gfs, double *k_even_gfs,
+                                double *y_nplus1_running_total_gfs, const double dt, MoL_graph **out);
+
 #ifdef __cplusplus
 }
 #endif

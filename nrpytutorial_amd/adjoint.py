@@ -133,6 +133,12 @@ def RK4Evolve():
     return _RK4Evolve
 
 
+def torch_mul(x, a, out):
+    """out = a * x in one device pass."""
+    import torch
+    torch.mul(x, a, out=out)
+
+
 class BSSNStepAdjoint:
     """Adjoint of the BSSN RK4 step on one Evolution, orchestrated from the host over the
     bundle's kernels. Each substep of the forward step is
@@ -169,6 +175,14 @@ class BSSNStepAdjoint:
             fn = getattr(L, f)
             fn.restype = ctypes.c_int
             fn.argtypes = [_VP] * 9
+        # index ranges of the Ricci tensor's inputs (hDD, lambdaU) among the evolved gridfunctions
+        idx = sorted(b.evol.index(n) for n in b.evol if n.startswith("hDD") or n.startswith("lambdaU"))
+        self.ricci_in = []
+        for i in idx:
+            if self.ricci_in and self.ricci_in[-1][1] == i:
+                self.ricci_in[-1] = (self.ricci_in[-1][0], i + 1)
+            else:
+                self.ricci_in.append((i, i + 1))
         L.bc_apply_T.restype = ctypes.c_int
         L.bc_apply_T.argtypes = [_VP, _VP, ctypes.c_int, _VP, _VP, _VP]
         L.rhs_eval_rk4_substep.restype = ctypes.c_int
@@ -181,12 +195,12 @@ class BSSNStepAdjoint:
         check(self.lib.bc_apply_T(ev.g.pp(), ev.g.bc, self.n, ev.g.bundle.evol_parity, ptr(c), ev.s()), "bc_apply_T")
 
     def _DT(self, P, c):
-        """c <- D'(P)^T c (the enforcement changes only hDD; other cotangents pass through)."""
+        """c <- D'(P)^T c (the enforcement changes only hDD; other cotangents pass through).
+        In place: pass 1 of the VJP reads every cotangent it needs into its slots before
+        pass 2 writes the hDD cotangents, and the two run in order on the stream."""
         ev = self.ev
-        self.tmp.copy_(c)
         check(self.lib.enforce_detgammabar_constraint_vjp(ev.g.rfm, ev.g.pp(), None, ptr(P), ptr(c), ptr(self.G),
-                                                          ptr(self.tmp), None, ev.s()), "detg vjp")
-        c.copy_(self.tmp)
+                                                          ptr(c), None, ev.s()), "detg vjp")
 
     def _RT(self, X, Ric, ck, out):
         """out <- (R + Ric)'(X)^T ck: full-array cotangent of the stage input X."""
@@ -194,10 +208,11 @@ class BSSNStepAdjoint:
         g = ev.g
         check(self.lib.rhs_eval_vjp(g.rfm, g.pp(), ptr(Ric), ptr(X), ptr(ck), ptr(self.G), ptr(out), ptr(self.cRic),
                                     ev.s()), "rhs_eval_vjp")
-        self.tmp.zero_()
+        # the Ricci VJP writes every point of the cotangents of hDD and lambdaU only
         check(self.lib.Ricci_eval_vjp(g.rfm, g.pp(), None, ptr(X), ptr(self.cRic), ptr(self.G), ptr(self.tmp), None,
                                       ev.s()), "Ricci_eval_vjp")
-        out.add_(self.tmp)
+        for a, b in self.ricci_in:
+            out[a:b].add_(self.tmp[a:b])
 
     def recompute(self, y_n, dt):
         ev, g, L = self.ev, self.ev.g, self.lib
@@ -223,7 +238,7 @@ class BSSNStepAdjoint:
         self.acc.copy_(c)
         w = (dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0)
         for s in range(4):
-            self.ck[s].copy_(c).mul_(w[s])
+            torch_mul(c, w[s], self.ck[s])
         coef = (dt / 2.0, dt / 2.0, dt)               # u_{s+1} = y_n + coef[s] * dt-weighted k_{s+1}
         for s in (3, 2, 1):                         # stage input X_s -> k_{s+1}
             self._RT(self.X[s], self.Ric[s], self.ck[s], self.cX)

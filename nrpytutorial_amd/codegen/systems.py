@@ -121,7 +121,34 @@ def _sommerfeld_glue(fd_order, rhs_call, **defaults):
     meta = {"fd_order": fd_order, "at_inf": list(bcs.vars_at_infinity.values()),
             "radial_falloff_power": list(bcs.vars_radial_falloff_power.values()),
             "speed": list(bcs.vars_speed.values())}
-    return "// One RK4 step with the Sommerfeld radiation condition on the RHS.\n" + bcs.hip_source() + step, meta
+    graph = (
+        "// The Sommerfeld RK4 step captured into a hipGraph (every launch is a kernel; pointers, xx\n"
+        "// and dt are fixed at capture); replay with MoL_graph_launch, free with MoL_graph_free.\n"
+        'extern "C" int MoL_sommerfeld_graph_create(const paramstruct *params, const rfm_struct *rfmstruct, '
+        "double *xx[3], const bc_struct *bcstruct, double *y_n_gfs, double *k_odd_gfs, double *k_even_gfs, "
+        "double *y_nplus1_running_total_gfs, const double dt, MoL_graph **out) {\n"
+        "  if (!out) return NRPY_ERR_NULL;\n  *out = nullptr;\n"
+        "  hipStream_t s;\n"
+        "  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return NRPY_ERR_LAUNCH;\n"
+        "  MoL_graph *g = new MoL_graph();\n"
+        "  int rc = NRPY_OK;\n"
+        "  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = NRPY_ERR_LAUNCH;\n"
+        "  if (!rc) {\n"
+        "    rc = MoL_step_forward_in_time_sommerfeld(params, rfmstruct, xx, bcstruct, y_n_gfs, k_odd_gfs, "
+        "k_even_gfs,\n                                             y_nplus1_running_total_gfs, dt, s);\n"
+        "    hipGraph_t gr;\n"
+        "    const hipError_t e = hipStreamEndCapture(s, &gr);\n"
+        "    if (!rc && e != hipSuccess) rc = NRPY_ERR_LAUNCH;\n"
+        "    if (!rc) {\n"
+        "      g->graph = gr;\n"
+        "      if (hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0) != hipSuccess) rc = NRPY_ERR_LAUNCH;\n"
+        "    }\n"
+        "  }\n"
+        "  (void)hipStreamDestroy(s);\n"
+        "  if (rc) { delete g; return rc; }\n"
+        "  *out = g;\n  return NRPY_OK;\n}\n")
+    return ("// One RK4 step with the Sommerfeld radiation condition on the RHS.\n" + bcs.hip_source() + step + graph,
+            meta)
 
 
 def _mol_batch_glue(n_ricci, n_rhs):
@@ -547,6 +574,11 @@ def maxwell_curvilinear(CoordSystem, bundle_name, NGHOSTS=3):
 BSSN_DIFFERENCED_GFS = "alpha,cf"
 
 
+def VJP_GROUPS(groups):
+    """Output groups of the BSSN RHS VJP's pass 1: the forward output groups."""
+    return [list(grp) for grp in groups]
+
+
 def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     from ..BSSN import BSSN_quantities as Bq, BSSN_RHSs as rhs, BSSN_gauge_RHSs as gaugerhs
     from ..BSSN import Enforce_Detgammabar_Constraint as EGC, BSSN_constraints as bssncon
@@ -709,11 +741,16 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     # Vector-Jacobian products of the whole RHS (all outputs, inputs in_gfs and RbarDD) and of
     # the Ricci tensor (inputs in_gfs), for adjoints through rhs_eval and Ricci_eval.
     t0 = time.time()
+    # Pass 1 runs per output group, as the forward kernels do; pass 2 per cotangent gridfunction.
+    vjp_groups = VJP_GROUPS(groups)
     vjp_src, vslots, _, _ = adj.vjp_kernels("rhs_eval", evol_rhss, betaU, "InteriorPoints,Enable_rfm_precompute",
-                                            "rfm_struct *restrict rfmstruct,")
+                                            "rfm_struct *restrict rfmstruct,",
+                                            groups=[[i for i, (n, _) in enumerate(pairs) if n in grp]
+                                                    for grp in vjp_groups], split_transpose=True)
     rvjp_src, rslots, _, _ = adj.vjp_kernels(
         "Ricci_eval", [lhrh(lhs=gri.gfaccess("auxevol_gfs", n), rhs=ric_exprs[n]) for n in ric_names], "",
-        "InteriorPoints,Enable_rfm_precompute", "rfm_struct *restrict rfmstruct,")
+        "InteriorPoints,Enable_rfm_precompute", "rfm_struct *restrict rfmstruct,",
+        groups=[[ric_names.index(n) for n in grp] for grp in ric_groups], split_transpose=True)
     # det(gammabar) enforcement and the Hamiltonian constraint, for the adjoint of the BSSN RK4
     # step and of the H-L2 diagnostic; the mass derivatives of the initial data.
     dvjp_src, dslots, _, _ = adj.vjp_kernels("enforce_detgammabar_constraint", detg_exprs, "",

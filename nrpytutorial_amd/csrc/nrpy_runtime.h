@@ -865,15 +865,15 @@ extern "C" int gridfunction_sumsq_noalloc(const paramstruct *params, const doubl
   do {                                                                                    \
     const double _r = (rhsval);                                                           \
     if ((S) == 0) {                                                                       \
-      rk_rt[nrpy_ix_##k] = _r * dt * (1.0 / 6.0);                                         \
-      rk_out[nrpy_ix_##k] = nrpy_y_##k + _r * dt * (1.0 / 2.0);                           \
+      rk_rt[nrpy_ix_##k] = _r * dt * (1.0 / 6.0);                                           \
+      rk_out[nrpy_ix_##k] = nrpy_y_##k + _r * dt * (1.0 / 2.0);                             \
     } else if ((S) == 1) {                                                                \
-      rk_rt[nrpy_ix_##k] = nrpy_r_##k + _r * dt * (1.0 / 3.0);                            \
-      rk_out[nrpy_ix_##k] = nrpy_y_##k + _r * dt * (1.0 / 2.0);                           \
+      rk_rt[nrpy_ix_##k] = nrpy_r_##k + _r * dt * (1.0 / 3.0);                              \
+      rk_out[nrpy_ix_##k] = nrpy_y_##k + _r * dt * (1.0 / 2.0);                             \
     } else if ((S) == 2) {                                                                \
-      rk_rt[nrpy_ix_##k] = nrpy_r_##k + _r * dt * (1.0 / 3.0);                            \
-      rk_out[nrpy_ix_##k] = nrpy_y_##k + _r * dt;                                         \
+      rk_rt[nrpy_ix_##k] = nrpy_r_##k + _r * dt * (1.0 / 3.0);                              \
+      rk_out[nrpy_ix_##k] = nrpy_y_##k + _r * dt;                                           \
     } else {                                                                              \
-      rk_out[nrpy_ix_##k] = nrpy_y_##k + nrpy_r_##k + _r * dt * (1.0 / 6.0);              \
+      rk_out[nrpy_ix_##k] = nrpy_y_##k + nrpy_r_##k + _r * dt * (1.0 / 6.0);                \
     }                                                                                     \
   } while (0)

@@ -84,7 +84,7 @@
 
 // Per-kernel block shapes, settable at run time (nrpy_set_block_shape) so an
 // autotuner can pick them; index by NRPY_KERNEL_* ids defined per bundle.
-#define NRPY_MAX_KERNELS 64
+#define NRPY_MAX_KERNELS 256  /* = loop.MAX_KERNEL_IDS */
 static int nrpy_block_shape[NRPY_MAX_KERNELS][3];
 static inline dim3 nrpy_block(int k, int dx, int dy, int dz) {
   if (k >= 0 && k < NRPY_MAX_KERNELS && nrpy_block_shape[k][0] > 0)

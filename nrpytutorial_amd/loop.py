@@ -40,8 +40,9 @@ def launch_bounds(name=None):
     import os
     w = os.environ.get("NRPY_MIN_WAVES_PER_EU", "") or MIN_WAVES_PER_EU.get(name, "")
     return "%d, %d" % (MAX_THREADS, int(w)) if w else "%d" % MAX_THREADS
-# kernel name -> id into the runtime's tunable block-shape table
+# kernel name -> id into the runtime's tunable block-shape table (NRPY_MAX_KERNELS entries)
 KERNEL_IDS = {}
+MAX_KERNEL_IDS = 256
 # kernel name -> what is needed to emit its batched (many-instance) variant
 BATCH_SPECS = {}
 
@@ -76,6 +77,9 @@ def stage_lds(name, kernel):
 
 def kernel_id(name):
     if name not in KERNEL_IDS:
+        if len(KERNEL_IDS) >= MAX_KERNEL_IDS:
+            raise ValueError("more than %d tunable kernels in one bundle: raise NRPY_MAX_KERNELS "
+                             "(csrc/nrpy_runtime_pre.h) and MAX_KERNEL_IDS together" % MAX_KERNEL_IDS)
         KERNEL_IDS[name] = len(KERNEL_IDS)
     return KERNEL_IDS[name]
 

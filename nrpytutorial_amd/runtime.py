@@ -80,6 +80,7 @@ PROTOTYPES = {
 SOMMERFELD_PROTOTYPES = {
     "apply_bcs_sommerfeld": (_I, [_VP, _DPP, _VP, _I, _VP, _VP, _VP, _VP]),
     "MoL_step_forward_in_time_sommerfeld": (_I, [_VP, _VP, _DPP, _VP, _VP, _VP, _VP, _VP, _D, _VP]),
+    "MoL_sommerfeld_graph_create": (_I, [_VP, _VP, _DPP, _VP, _VP, _VP, _VP, _VP, _D, ctypes.POINTER(_VP)]),
 }
 KERNEL_PROTOTYPES = {
     # reference-style kernels: (params..., stream)
@@ -292,6 +293,8 @@ class Evolution:
         self.workspace = grid.torch.zeros(1024 + 1, dtype=grid.torch.float64, device="cuda")
         self.graph = None
         self.graph_dt = None
+        self.som_graph = None
+        self.som_graph_dt = None
 
     def s(self):
         return stream_handle(self.g.torch)
@@ -336,8 +339,20 @@ class Evolution:
                                             self.g.bundle.evol_parity, ptr(gfs), ptr(rhs_gfs), self.s()),
               "apply_bcs_sommerfeld")
 
-    def step_sommerfeld(self, dt, nsteps=1):
-        """nsteps RK4 steps with the Sommerfeld condition on the RHS and the update at every point."""
+    def step_sommerfeld(self, dt, nsteps=1, use_graph=True):
+        """nsteps RK4 steps with the Sommerfeld condition on the RHS and the update at every point,
+        replayed from a captured hipGraph (or launched eagerly)."""
+        if use_graph:
+            if self.som_graph is None or self.som_graph_dt != dt:
+                self.free_graph()
+                h = ctypes.c_void_p()
+                check(self.lib.MoL_sommerfeld_graph_create(self.g.pp(), self.g.rfm, self.g.xx_dev_ptrs, self.g.bc,
+                                                           ptr(self.y_n), ptr(self.k_odd), ptr(self.k_even),
+                                                           ptr(self.rt), float(dt), ctypes.byref(h)),
+                      "MoL_sommerfeld_graph_create")
+                self.som_graph, self.som_graph_dt = h, dt
+            check(self.lib.MoL_graph_launch(self.som_graph, int(nsteps), self.s()), "MoL_graph_launch")
+            return
         for _ in range(nsteps):
             check(self.lib.MoL_step_forward_in_time_sommerfeld(self.g.pp(), self.g.rfm, self.g.xx_dev_ptrs, self.g.bc,
                                                                ptr(self.y_n), ptr(self.k_odd), ptr(self.k_even),
@@ -368,6 +383,9 @@ class Evolution:
         if self.graph is not None:
             self.lib.MoL_graph_free(self.graph)
             self.graph = None
+        if self.som_graph is not None:
+            self.lib.MoL_graph_free(self.som_graph)
+            self.som_graph = None
 
     def __del__(self):
         try:

@@ -85,8 +85,14 @@ def test_block_shape_setter_validates():
     assert f(0, 0, 0, 0) == 0
     assert f(0, 64, 2, 4) == -4       # 512 threads
     assert f(0, 48, 1, 1) == -4       # not a multiple of 64
-    assert f(99, 64, 1, 1) == -4
+    assert f(256, 64, 1, 1) == -4     # beyond the table (NRPY_MAX_KERNELS = 256)
     assert f(-1, 64, 1, 1) == -4
+    # every kernel id a bundle hands out fits the table, so tuning reaches every kernel
+    for name in BUNDLES:
+        ids = rt.load_meta(name).get("kernel_ids", {})
+        assert all(0 <= k < 256 for k in ids.values()), name
+    assert f(max(rt.load_meta("bssn_spherical_sym2")["kernel_ids"].values()), 64, 4, 1) == 0
+    assert f(max(rt.load_meta("bssn_spherical_sym2")["kernel_ids"].values()), 0, 0, 0) == 0
 
 
 def test_null_and_shape_errors_on_host():

@@ -100,3 +100,24 @@ def test_mol_sommerfeld_matches_oracle(torch_cuda, bundle, pars):
     at = np.unravel_index(np.argmax(np.abs(got - y)), y.shape)
     print("%s: 10 RK4 steps with Sommerfeld, rel err %.2e at (gf, i2, i1, i0) = %s" % (bundle, err, at))
     assert err < (TOL_EVOL_CURVI if "cartesian" not in bundle else TOL_EVOL), err
+
+
+@pytest.mark.parametrize("bundle,pars", [CASES[1], CASES[-1]], ids=[CASES[1][0], CASES[-1][0]])
+def test_sommerfeld_graph_replays_equal_eager_steps(torch_cuda, bundle, pars):
+    """MoL_sommerfeld_graph_create: each replay of the captured step equals an eager step bit for
+    bit, replay after replay, with eager work and synchronizations in between."""
+    torch = torch_cuda
+    from nrpytutorial_amd.runtime import Evolution
+    Nxx = (20, 18, 16)
+    print("case: %s Sommerfeld graph replays vs eager steps Nxx=%s" % (bundle, Nxx), flush=True)
+    b, g, ev, og, y = _setup(torch, bundle, pars, Nxx)
+    ev2 = Evolution(g)
+    ev2.y_n.copy_(ev.y_n)
+    dt = 0.5 * g.params.dxx0
+    for rep in range(3):
+        ev.step_sommerfeld(dt, 2, use_graph=True)
+        torch.cuda.synchronize()
+        ev2.step_sommerfeld(dt, 2, use_graph=False)
+        torch.cuda.synchronize()
+        assert torch.equal(ev.y_n, ev2.y_n), "replay %d differs from eager steps" % rep
+    ev.free_graph()
