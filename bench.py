@@ -38,54 +38,88 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def physical_cores(limit=16):
+    """One logical CPU per physical core among those this process may use (sibling
+    hyperthreads dropped), at most ``limit``: the reference pins its OpenMP runs with
+    taskset to cpu_count/2 cores when SMT is on (cmdline_helper.py:127-145)."""
+    seen, picked = set(), []
+    for c in sorted(os.sched_getaffinity(0)):
+        try:
+            with open("/sys/devices/system/cpu/cpu%d/topology/core_id" % c) as f:
+                core = f.read().strip()
+            with open("/sys/devices/system/cpu/cpu%d/topology/physical_package_id" % c) as f:
+                pkg = f.read().strip()
+        except OSError:
+            core, pkg = str(c), "0"
+        if (pkg, core) not in seen:
+            seen.add((pkg, core))
+            picked.append(c)
+    return picked[:limit]
+
+
+def cpu_model():
+    """'model name' from /proc/cpuinfo plus the host's socket/core/thread counts."""
+    model, sockets, logical = "", set(), 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and not model:
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("physical id"):
+                    sockets.add(line.split(":", 1)[1].strip())
+                elif line.startswith("processor"):
+                    logical += 1
+    except OSError:
+        pass
+    return "%s (%d socket(s), %d logical CPUs on the host)" % (model or "unknown", max(1, len(sockets)), logical)
+
+
 def cpu_baseline_worker(n, min_seconds, max_steps):
-    """Runs in a child process (fresh OpenMP runtime): oracle RK4 steps on an n^3 grid."""
+    """Runs in a child process (fresh OpenMP runtime, pinned by the parent): oracle RK4 steps of
+    the benchmark itself, the seeded flat + Gaussian BSSN state on the n^3 Cartesian grid."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
+    import torch
     from oracle_lib import Oracle, OracleGrid
-    from nrpytutorial_amd.workloads import bssn_gaussian_state_numpy
+    from nrpytutorial_amd.workloads import bssn_gaussian_state_torch
     og = OracleGrid(Oracle("bssn_cartesian", fast=True), (n, n, n), {})
-    y0 = bssn_gaussian_state_numpy(og.xx, og.o.evol, 1)
+    y0 = np.zeros((len(og.o.evol),) + og.shape)
+    bssn_gaussian_state_torch(torch.from_numpy(y0), [torch.from_numpy(x) for x in og.xx], og.o.evol, 1)
     og.apply_bcs(y0)
     og.enforce_detg(y0)
     dt = 0.5 * og.p.dxx0
     y = y0.copy()
     og.mol_step(y, dt, 1)  # warm-up
     steps, el = 0, 0.0
-    # The synthetic data are re-seeded every 10 steps (outside the timer) so the
-    # sample stays a short evolution of the benchmark state.
     while steps < max_steps and el < min_seconds:
         y[...] = y0
         t0 = time.perf_counter()
-        og.mol_step(y, dt, 10)
+        og.mol_step(y, dt, 2)
         el += time.perf_counter() - t0
-        steps += 10
+        steps += 2
         if not np.isfinite(y).all():
             raise RuntimeError("oracle state became non-finite")
     return {"gp_per_s": 4.0 * n ** 3 * steps / el, "steps": steps, "seconds": el}
 
 
-def cpu_baseline(n=64, min_seconds=12.0, max_steps=200):
-    cores = int(os.environ.get("NRPY_CPU_BASELINE_THREADS", min(16, os.cpu_count() or 1)))
+def cpu_baseline(n=192, min_seconds=10.0, max_steps=8):
+    """The C oracle built with the reference's optimized flags, timed on the host on the
+    benchmark's own workload (n^3, seeded state, RK4 steps of dt = 0.5 dx), one OpenMP
+    thread per physical core, pinned."""
+    cpus = physical_cores(int(os.environ.get("NRPY_CPU_BASELINE_THREADS", 16)))
+    cores = len(cpus)
     env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", str(n),
-                        str(min_seconds), str(max_steps)], env=env, capture_output=True, text=True, timeout=600)
+                        str(min_seconds), str(max_steps)], env=env, capture_output=True, text=True, timeout=900,
+                       preexec_fn=lambda: os.sched_setaffinity(0, cpus))
     if r.returncode != 0:
         raise RuntimeError("cpu baseline failed: " + r.stderr[-2000:])
     res = json.loads(r.stdout.strip().splitlines()[-1])
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
     return {"value": res["gp_per_s"], "unit": "gridpoint-updates/s", "cores": cores, "kind": "port",
-            "sample": "C oracle (gcc -Ofast -fopenmp -march=native -funroll-loops), BSSN Cartesian %d^3 seeded "
-                      "Gaussians, %d RK4 steps in %.1f s, OMP_NUM_THREADS=%d, CPU: %s"
-                      % (n, res["steps"], res["seconds"], cores, model)}
+            "sample": "C oracle (gcc -Ofast -fopenmp -march=native -funroll-loops) on the benchmark workload: BSSN "
+                      "Cartesian %d^3 seeded flat + Gaussians, %d RK4 steps (dt = 0.5 dx) in %.1f s, OMP_NUM_THREADS=%d "
+                      "pinned to CPUs %s (one per physical core), CPU: %s"
+                      % (n, res["steps"], res["seconds"], cores, ",".join(map(str, cpus)), cpu_model())}
 
 
 def measured_traffic(kernel, n, outputs):
