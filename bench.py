@@ -438,6 +438,43 @@ def bench_etk(torch, ev, reps, t_ricci, t_rhs):
             "equal_to_senr_kernels": same}
 
 
+def summarize_trace(csv_path, last=0, match=None, top=20):
+    """Text summary of a rocprofv3 --kernel-trace CSV (no GPU needed): over the kernels whose
+    name matches the regular expression `match` (all if None), and of those the last `last`
+    (0 = all), per-kernel calls, average and total duration, the window, the time the GPU
+    ran a kernel (overlaps merged) and the gaps between consecutive kernels."""
+    import csv
+    import re
+    rows = []
+    with open(csv_path) as f:
+        for r in csv.DictReader(f):
+            if match is None or re.search(match, r["Kernel_Name"]):
+                rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    if last:
+        rows = rows[-last:]
+    agg = {}
+    for t0, t1, name in rows:
+        a = agg.setdefault(name, [0, 0])
+        a[0] += 1
+        a[1] += t1 - t0
+    busy, gaps, end = 0, [], rows[0][0]
+    for t0, t1, _ in rows:
+        if t0 > end:
+            gaps.append(t0 - end)
+        busy += max(0, t1 - max(t0, end))
+        end = max(end, t1)
+    win = end - rows[0][0]
+    gaps += [0] * (len(rows) - 1 - len(gaps))
+    gaps.sort()
+    out = ["%d kernels: window %.3f ms, GPU busy %.3f ms (%.1f%%), gap between kernels mean %.2f us, "
+           "median %.2f us" % (len(rows), win / 1e6, busy / 1e6, 100.0 * busy / win,
+                               sum(gaps) / max(len(gaps), 1) / 1e3, gaps[len(gaps) // 2] / 1e3 if gaps else 0.0)]
+    for name, (n, tot) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append("  %-70s n=%5d avg %9.1f us  total %9.2f ms" % (name[:70], n, tot / n / 1e3, tot / 1e6))
+    return "\n".join(out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -459,7 +496,16 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device copy (scale for the HBM byte counters)")
     ap.add_argument("--cpu-baseline-worker", nargs=3)
+    ap.add_argument("--summarize-trace", metavar="CSV",
+                    help="print a summary of a rocprofv3 kernel-trace CSV and exit (no GPU used)")
+    ap.add_argument("--trace-last", type=int, default=0,
+                    help="with --summarize-trace: only the last N (matching) kernels of the trace")
+    ap.add_argument("--trace-match", metavar="REGEX",
+                    help="with --summarize-trace: only kernels whose name matches REGEX")
     a = ap.parse_args()
+    if a.summarize_trace:
+        print(summarize_trace(a.summarize_trace, a.trace_last, a.trace_match))
+        return 0
     if a.cpu_baseline_worker:
         n, sec, mx = a.cpu_baseline_worker
         print(json.dumps(cpu_baseline_worker(int(n), float(sec), int(mx))))
