@@ -413,7 +413,7 @@ def bench_etk(torch, ev, reps, t_ricci, t_rhs):
     each gridfunction its own array (the slices of that state: cctk_ash = cctk_lsh, three
     ghost zones), one RHS evaluation = Baikal_driver_pt1_BSSN_Ricci + Baikal_driver_pt2_BSSN_RHSs,
     against Ricci_eval + rhs_eval of the SENR layout on the same state. Also checks that
-    both forms give the same values."""
+    both forms give the same values (to rounding for the lane-move group, see below)."""
     from nrpytutorial_amd.etk import Baikal, cctk_grid
     g = ev.g
     G = g.bundle.NGHOSTS
@@ -429,13 +429,24 @@ def bench_etk(torch, ev, reps, t_ricci, t_rhs):
     t_r = time_kernel(torch, lambda: k.ricci(gh, params, evol, rbar_l, s), reps)
     t_h = time_kernel(torch, lambda: k.rhss(gh, params, evol, rbar_l, rhs_l, s), reps)
     torch.cuda.synchronize()
-    same = bool(torch.equal(g.interior(rbar), g.interior(ev.auxevol)) and
-                torch.equal(g.interior(rhs), g.interior(ev.k_even)))
+    # bit for bit, except the outputs of SENR kernels that build their x-neighbours by lane
+    # moves (bundle meta "xshift_kernels"): the compiler contracts their reordered statements
+    # into FMAs differently, so those agree to rounding (1e-13 of each gridfunction's max norm)
+    b = g.bundle
+    moved = {n for q, grp in enumerate(b.meta["rhs_groups"])
+             if "rhs_eval_g%d" % q in b.meta.get("xshift_kernels", []) for n in grp}
+    ri, rs = g.interior(rhs), g.interior(ev.k_even)
+    same = bool(torch.equal(g.interior(rbar), g.interior(ev.auxevol)))
+    for q, gf in enumerate(b.evol):
+        if gf in moved:
+            same = same and float((ri[q] - rs[q]).abs().max()) <= 1e-13 * float(rs[q].abs().max())
+        else:
+            same = same and bool(torch.equal(ri[q], rs[q]))
     npts = g.Nxx[0] * g.Nxx[1] * g.Nxx[2]
     return {"grid": list(g.Nxx), "ricci_ms": 1e3 * t_r, "rhss_ms": 1e3 * t_h,
             "senr_ricci_ms": 1e3 * t_ricci, "senr_rhs_eval_ms": 1e3 * t_rhs,
             "rhs_evals_gp_per_s": npts / (t_r + t_h), "senr_rhs_evals_gp_per_s": npts / (t_ricci + t_rhs),
-            "equal_to_senr_kernels": same}
+            "matches_senr_kernels": same}
 
 
 def summarize_trace(csv_path, last=0, match=None, top=20):
@@ -591,7 +602,7 @@ def main():
         log("ETK plugin form %d^3: Ricci %.3f ms + RHSs %.3f ms (SENR layout %.3f + %.3f ms), "
             "%.3e RHS-evaluation gp/s; same values as the SENR kernels: %s"
             % (N, etk["ricci_ms"], etk["rhss_ms"], etk["senr_ricci_ms"], etk["senr_rhs_eval_ms"],
-               etk["rhs_evals_gp_per_s"], etk["equal_to_senr_kernels"]))
+               etk["rhs_evals_gp_per_s"], etk["matches_senr_kernels"]))
     if a.calibrate_copy:
         # A plain device copy of known size, for scaling the HBM byte counters.
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)

@@ -194,3 +194,194 @@ def stage_kernel(kernel_src, nghosts, tile=TILE):
             continue
         out.append(line)
     return "\n".join(out)
+
+
+# ---------------------------------------------------------------------------
+# x-neighbours by cross-lane moves
+# ---------------------------------------------------------------------------
+# Every stencil read of a gridfunction that is not staged as a box is a global load,
+# and these kernels spend most of their time in the texture addresser (tens of cycles
+# per 64-lane load). An x-offset read (i0 +- 1..3 at the same i1, i2) loads a value
+# that a neighbouring lane of the same row holds as its centre value. This pass keeps
+# two loads per such gridfunction, the centre value C and one halo value H (the first
+# HX lanes of a row load i0 - HX, the others i0 + HX), and builds the x-offset values
+# with DPP lane moves (nrpy_xshift in nrpy_runtime_pre.h): f(i0 + k) is C of lane
+# l + k of the row, and for the lanes at the row's ends whose neighbour lies outside
+# it, H of the lane that loaded exactly that point. A row is TX = 16 lanes
+# (row_shl/row_shr) or the whole 64-lane wave (wave_shl/wave_shr by one lane, chained).
+#
+# A lane move reads nothing from a lane that has exited, so every row must have all
+# its lanes, or at least 2 HX of them when the grid is narrower than one row. The last
+# tile along x is therefore moved left to end at the grid's last point and overlaps
+# the tile before it; its lanes over points of that tile compute them again and send
+# their stores to a discard array (a pointer select, NRPY_OWNED: a branch around the
+# stores would let the compiler move the whole computation behind it, while the lane
+# moves, which cannot enter divergent code, stay in front). The in-place RK4 update
+# (the output aliases y_n in the last substep) stays race-free: every point is stored
+# by exactly one lane. Values are unchanged, so results are bit-identical; the bundle
+# rejects grids with fewer than MIN_NXX0 points along x.
+
+# kernels (names) whose x-neighbours are built by lane moves; NRPY_XSHIFT (comma list or
+# "all") and NRPY_XSHIFT_SINK=0 override for measurements
+XSHIFT_KERNELS = set()
+MIN_NXX0 = 0          # set when a kernel is rewritten: 2 HX
+
+
+def xshift_wanted(name):
+    import os
+    env = os.environ.get("NRPY_XSHIFT")
+    if env is not None:
+        return env == "all" or name in env.split(",")
+    return name in XSHIFT_KERNELS
+
+
+def xshift_sink():
+    import os
+    return os.environ.get("NRPY_XSHIFT_SINK", "1") != "0"
+_STORE = re.compile(r"^(\s*)(\w+)\[(IDX4\(\w+, i0, i1, i2\))\] = (.*);$")
+_I0 = re.compile(r"^(\s*)const int i0 = NGHOSTS \+ nrpy_bx\*\(int\)blockDim\.x \+ \(int\)threadIdx\.x;$")
+
+
+def xshift_kernel(kernel_src, nghosts, tile):
+    """Kernel text with the x-offset stencil reads of its unstaged gridfunctions built from
+    the centre and one halo load by cross-lane moves, or None when nothing qualifies.
+    The result must be launched with blockDim = tile."""
+    global MIN_NXX0
+    TX = tile[0]
+    if TX not in (16, 64) or (tile[0] * tile[1] * tile[2]) % 64:
+        return None
+    lines = kernel_src.split("\n")
+    bounds = [k for k, line in enumerate(lines) if _BOUNDS.match(line)]
+    i0line = [k for k, line in enumerate(lines) if _I0.match(line)]
+    if len(bounds) != 1 or len(i0line) != 1:
+        return None
+    reads, order = {}, []
+    for line in lines:
+        m = _READ.match(line)
+        if not m:
+            continue
+        gf = m.group(3)
+        if gf not in reads:
+            reads[gf] = (m.group(2), set())
+            order.append(gf)
+        reads[gf][1].add(tuple(int(x) if x else 0 for x in m.group(4, 5, 6)))
+    hx = {}
+    for g in order:
+        xs = [abs(d[0]) for d in reads[g][1] if d[0] and not d[1] and not d[2]]
+        if len(xs) >= 2:
+            hx[g] = max(xs)
+    if not hx or max(hx.values()) > nghosts:
+        return None
+    MIN_NXX0 = max(MIN_NXX0, 2 * max(hx.values()))
+    q_of = {g: q for q, g in enumerate(g for g in order if g in hx)}
+    out, done, moved = [], set(), []
+    for k, line in enumerate(lines):
+        if k == i0line[0]:
+            ind = _I0.match(line).group(1)
+            out += [ind + "const int nrpy_xt = nrpy_bx * %d;  /* first x of this tile's own points */" % TX,
+                    ind + "const int nrpy_x0t = Nxx0 >= %d ? min(nrpy_xt, Nxx0 - %d) : 0;  /* the tile, moved to fit */"
+                    % (TX, TX),
+                    ind + "const int i0 = NGHOSTS + nrpy_x0t + (int)threadIdx.x;",
+                    ind + "const bool nrpy_own = nrpy_x0t + (int)threadIdx.x >= nrpy_xt;  /* else: the previous tile's point */",
+                    ind + "const int nrpy_xl = (int)threadIdx.x;  /* lane within the row */",
+                    ind + "const int nrpy_xm = min(%d, Nxx0);  /* lanes in the row */" % TX]
+            continue
+        if "const int x0_ = nrpy_bx * %d," % TX in line:      # the staged box starts at the moved tile
+            out.append(line.replace("const int x0_ = nrpy_bx * %d," % TX, "const int x0_ = nrpy_x0t,"))
+            continue
+        if line.lstrip().startswith("if (%s && " % _SENR["even"]):  # 16-byte copy: also an even box start
+            out.append(line.replace("if (%s && " % _SENR["even"], "if (%s && (x0_ & 1) == 0 && " % _SENR["even"]))
+            continue
+        sm = _STORE.match(line)
+        if sm:
+            out.append("%s*NRPY_OWNED(&%s[%s]) = %s;" % sm.group(1, 2, 3, 4))
+            continue
+        if line.strip().startswith("NRPY_RK4_STORE("):
+            out.append(line.replace("NRPY_RK4_STORE(", "NRPY_RK4_STORE_OWNED("))
+            continue
+        m = _READ.match(line)
+        if m and m.group(3) in q_of:
+            gf, arr = m.group(3), m.group(2)
+            q, H = q_of[gf], hx[gf]
+            ind = re.match(r"^(\s*)", line).group(1)
+            if q not in done:
+                done.add(q)
+                out.append("%sconst double nrpy_c%d = %s[IDX4(%s, i0,i1,i2)];" % (ind, q, arr, gf))
+                out.append("%sconst double nrpy_h%d = %s[IDX4(%s, nrpy_xl < %d ? i0 - %d : i0 + %d,i1,i2)];"
+                           % (ind, q, arr, gf, H, H, H))
+            d = tuple(int(x) if x else 0 for x in m.group(4, 5, 6))
+            if d == (0, 0, 0):
+                out.append("%snrpy_c%d;" % (m.group(1), q))
+                continue
+            if not d[1] and not d[2]:
+                moved.append((re.match(r"\s*const double (\w+) =", line).group(1),
+                              "%snrpy_xshift<%d, %d, %d>(nrpy_c%d, nrpy_h%d, nrpy_xl, nrpy_xm);"
+                              % (m.group(1), TX, H, d[0], q, q)))
+                continue
+        out.append(line)
+    # each moved value is built right before the first statement that uses it
+    pending = dict(moved)
+    final = []
+    for line in out:
+        if pending:
+            for name in [n for n in pending if re.search(r"\b%s\b" % n, line)]:
+                final.append(pending.pop(name))
+        final.append(line)
+    if pending or any("i0" in l and "nrpy_bx*(int)blockDim.x" in l for l in final):
+        return None
+    return "\n".join(final)
+
+
+# ---------------------------------------------------------------------------
+# statement order: values built by cross-lane moves right before their first use
+# ---------------------------------------------------------------------------
+# The finite-difference emitter writes every stencil read and derivative first and
+# the right-hand sides after them. The compiler spreads global loads towards their
+# uses by itself, but not cross-lane moves: those stay where the source puts them,
+# and a value built at the top stays live to its use. This pass moves the values
+# built by nrpy_xshift, and the stencil-level definitions computed from them
+# (derivatives, upwinded derivatives), down to just before the first statement of
+# their block that uses them (the last statements first, so a moved definition's
+# operands follow it down). Only these pure definitions move, and results are
+# unchanged. (Moving every definition this way raises register use: the compiler's
+# own order is better for the rest.)
+
+_DEF = re.compile(r"^\s*const double (\w+) = [^;]*;$")
+_IDENT = re.compile(r"[A-Za-z_]\w*")
+
+
+def sink_xshift_values(kernel_src):
+    lines = kernel_src.split("\n")
+    block, bid = [], 0
+    for line in lines:
+        if "{" in line or "}" in line:
+            bid += 1
+        block.append(bid)
+    defs = {}
+    for k, line in enumerate(lines):
+        m = _DEF.match(line)
+        if m:
+            defs[k] = m.group(1)
+    # the x-values and every stencil-level definition (before the right-hand-side step)
+    # computed from them, transitively
+    rhs_step = min([k for k, line in enumerate(lines) if "evaluate right-hand sides" in line or
+                    "Step 3 of 3" in line or "Step 2 of 2" in line] or [len(lines)])
+    xvals = {defs[k] for k in defs if "nrpy_xshift<" in lines[k]}
+    movable = set()
+    for k in sorted(defs):
+        if k < rhs_step and (defs[k] in xvals or xvals & set(_IDENT.findall(lines[k].split("=", 1)[1]))):
+            movable.add(k)
+            xvals.add(defs[k])
+    uses = {}
+    for k, line in enumerate(lines):
+        for name in set(_IDENT.findall(line)):
+            uses.setdefault(name, []).append(k)
+    key = {k: float(k) for k in range(len(lines))}
+    for k in sorted(movable, reverse=True):
+        later = [u for u in uses.get(defs[k], []) if u > k]
+        if not later:
+            continue
+        u = min(later, key=lambda x: key[x])
+        if block[u] == block[k]:
+            key[k] = key[u] - 1e-9 * (1 + k)
+    return "\n".join(lines[k] for k in sorted(range(len(lines)), key=lambda k: key[k]))

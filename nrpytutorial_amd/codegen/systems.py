@@ -31,6 +31,7 @@ from .. import reference_metric as rfm
 from .. import finite_difference as fin
 from .. import outputC as oc
 from ..outputC import lhrh
+from . import lds
 from . import bundle as bd
 from . import adjoint as adj
 from . import batch as bt
@@ -322,6 +323,11 @@ DEFAULT_RICCI_GROUPS = "RbarDD00,RbarDD01,RbarDD02|RbarDD11,RbarDD12,RbarDD22"
 # 2.30 ms: their larger boxes take longer to copy).
 ROW_TILE = (64, 1, 4)
 ROW_TILE_OUTPUTS = {"aDD00", "aDD01", "aDD02", "aDD11", "aDD12", "aDD22", "trK"}
+# The kernels of this output group build their x-neighbours by lane moves (codegen/lds.py):
+# with them the group runs at three waves per SIMD, which hides the moves' latency
+# (1.22 -> 1.03 ms per substep at 192^3). The other groups and the Ricci kernels, at one
+# or two waves per SIMD, were measured slower that way (2.34 -> 2.95, 1.15 -> 1.42 ms).
+XSHIFT_OUTPUTS = {"hDD00", "hDD01", "hDD02", "hDD11", "hDD12", "hDD22", "alpha", "cf"}
 
 
 def lds_tile_for_group(grp):
@@ -638,12 +644,16 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     # curvilinear grids of the Brill-Lindquist runs (two points in phi) keep the
     # plain kernels.
     bd.lp.LDS_STAGING = (FD_order // 2 + 1) if CoordSystem == "Cartesian" else 0
+    lds.MIN_NXX0 = 0
+    lds.XSHIFT_KERNELS = set()
     if CoordSystem == "Cartesian":
         # the aDD,trK group needs a few registers more than two waves per SIMD allow once its
         # lapse and conformal-factor stencils are differenced; asking for two waves fits it
         # (no spills) and keeps its occupancy
         bd.lp.MIN_WAVES_PER_EU["rhs_eval_rk4_substep_g0"] = bd.lp.MIN_WAVES_PER_EU["rhs_eval_g0"] = 2
     for k, grp in enumerate(groups):
+        if bd.lp.LDS_STAGING and set(grp) == XSHIFT_OUTPUTS:
+            lds.XSHIFT_KERNELS |= {"rhs_eval_g%d" % k, "rhs_eval_rk4_substep_g%d" % k}
         if bd.lp.LDS_STAGING and lds_tile_for_group(grp):
             bd.lp.LDS_TILES["rhs_eval_g%d" % k] = bd.lp.LDS_TILES["rhs_eval_rk4_substep_g%d" % k] = \
                 lds_tile_for_group(grp)
@@ -779,6 +789,7 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     meta["hamiltonian_vjp_slots"] = [sl[0] for sl in hslots]
     meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
     meta["lds_kernels"] = dict(bd.lp.LDS_KERNELS)
+    meta["xshift_kernels"] = sorted(lds.XSHIFT_KERNELS)
     meta["cost"] = {"rhs_eval": {"flops": sum(c["flops"] for c in group_costs), "groups": group_costs},
                     "Ricci_eval": {"flops": sum(c["flops"] for c in ricci_costs), "groups": ricci_costs},
                     "Hamiltonian_constraint": bd.count_body(h_body),

@@ -877,3 +877,20 @@ extern "C" int gridfunction_sumsq_noalloc(const paramstruct *params, const doubl
       rk_out[nrpy_ix_##k] = nrpy_y_##k + nrpy_r_##k + _r * dt * (1.0 / 6.0);                \
     }                                                                                     \
   } while (0)
+// The same for kernels in which some lanes recompute points other lanes own (NRPY_OWNED).
+#define NRPY_RK4_STORE_OWNED(S, k, rhsval)                                                   \
+  do {                                                                                    \
+    const double _r = (rhsval);                                                           \
+    if ((S) == 0) {                                                                       \
+      *NRPY_OWNED(&rk_rt[nrpy_ix_##k]) = _r * dt * (1.0 / 6.0);                                           \
+      *NRPY_OWNED(&rk_out[nrpy_ix_##k]) = nrpy_y_##k + _r * dt * (1.0 / 2.0);                             \
+    } else if ((S) == 1) {                                                                \
+      *NRPY_OWNED(&rk_rt[nrpy_ix_##k]) = nrpy_r_##k + _r * dt * (1.0 / 3.0);                              \
+      *NRPY_OWNED(&rk_out[nrpy_ix_##k]) = nrpy_y_##k + _r * dt * (1.0 / 2.0);                             \
+    } else if ((S) == 2) {                                                                \
+      *NRPY_OWNED(&rk_rt[nrpy_ix_##k]) = nrpy_r_##k + _r * dt * (1.0 / 3.0);                              \
+      *NRPY_OWNED(&rk_out[nrpy_ix_##k]) = nrpy_y_##k + _r * dt;                                           \
+    } else {                                                                              \
+      *NRPY_OWNED(&rk_out[nrpy_ix_##k]) = nrpy_y_##k + nrpy_r_##k + _r * dt * (1.0 / 6.0);                \
+    }                                                                                     \
+  } while (0)

@@ -72,6 +72,47 @@
     nrpy_bz = t_ / (nbx_ * nby_);                                                              \
   }
 
+// x-neighbours of a stencil by cross-lane moves (DPP). A row is TX lanes holding
+// consecutive i0 of one (i1, i2): 16 lanes (row_shl:k / row_shr:k move a value k lanes
+// within each 16-lane row) or the whole 64-lane wave (wave_shl:1 / wave_shr:1, chained).
+// The row's first m lanes are live (m = TX, or the grid's width when it is narrower,
+// m >= 2 HX). Lane l holds its centre value c = f(i0) and a halo value h: f(i0 - HX) in
+// the first HX lanes, f(i0 + HX) in the others. nrpy_xshift<TX, HX, K>(c, h, l, m)
+// returns f(i0 + K), |K| <= HX: c of lane l + K, or, where that lane is outside the live
+// part of the row, h of the lane HX - |K| lanes back towards the row's middle, which
+// loaded exactly that point.
+template <int CTRL>
+__device__ __forceinline__ double nrpy_dpp(const double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+// lane l receives v of lane l + K (K > 0) or l - |K| (K < 0) of its row
+template <int TX, int K>
+__device__ __forceinline__ double nrpy_lane_move(const double v) {
+  if constexpr (K == 0) {
+    return v;
+  } else if constexpr (TX == 16) {
+    return nrpy_dpp<(K > 0 ? 0x100 + K : 0x110 - K)>(v);          /* row_shl:K / row_shr:-K */
+  } else {
+    static_assert(TX == 64, "rows are 16 lanes or the whole wave");
+    return nrpy_dpp<(K > 0 ? 0x130 : 0x138)>(nrpy_lane_move<64, (K > 0 ? K - 1 : K + 1)>(v));  /* wave_shl:1 / wave_shr:1 */
+  }
+}
+template <int TX, int HX, int K>
+__device__ __forceinline__ double nrpy_xshift(const double c, const double h, const int l, const int m) {
+  static_assert(K != 0 && (K > 0 ? K : -K) <= HX && 2 * HX <= TX, "x offset out of the halo");
+  // both moves run in every lane, before the select: a lane move inside a branch would
+  // read the lanes that did not take the branch as 0
+  const double in_row = nrpy_lane_move<TX, K>(c);
+  const double halo = nrpy_lane_move<TX, (K > 0 ? K - HX : K + HX)>(h);
+  return ((K > 0) ? (l + K < m) : (l + K >= 0)) ? in_row : halo;
+}
+// Lanes that recompute a point another lane owns store into this array instead
+// (a pointer select, so the computation stays free of branches).
+static __device__ double nrpy_discard[64];
+#define NRPY_OWNED(p) (nrpy_own ? (p) : &nrpy_discard[threadIdx.x & 63])
+
 // Batched launches (nrpy_batch.h): array roles of an instance and index ranges.
 #define NRPY_BATCH_ROLE_YN 0
 #define NRPY_BATCH_ROLE_KODD 1

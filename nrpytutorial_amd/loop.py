@@ -62,13 +62,23 @@ def lds_tile(name):
 
 
 def stage_lds(name, kernel):
-    """(kernel text, launcher block expression or None): the LDS-staged kernel when
-    LDS_STAGING is on and the kernel has mixed-derivative stencils."""
+    """(kernel text, launcher block expression or None): when LDS_STAGING is on, the kernel
+    with its mixed-derivative stencils staged in LDS boxes and the x-offset reads of its
+    other stencil gridfunctions built by cross-lane moves (codegen/lds.py). A kernel with
+    neither keeps its tunable block shape; one with only the moves runs on the default
+    block shape, fixed."""
     if not LDS_STAGING:
         return kernel, None
     from .codegen import lds
     tile = lds_tile(name)
     staged = lds.stage_kernel(kernel, LDS_STAGING, tile)
+    if staged is None:
+        tile = tuple(LDS_TILES.get(name, (BLOCK_X, BLOCK_Y, BLOCK_Z)))
+    shifted = None
+    if lds.xshift_wanted(name):
+        shifted = lds.xshift_kernel(staged if staged is not None else kernel, LDS_STAGING, tile)
+    if shifted is not None:
+        staged = lds.sink_xshift_values(shifted) if lds.xshift_sink() else shifted
     if staged is None:
         return kernel, None
     LDS_KERNELS[name] = list(tile)

@@ -7,7 +7,10 @@ seeded flat + Gaussian state of the SENR-layout tests:
 * Ricci tensor, BSSN right-hand sides, H and MU at the interior points
   [cctk_nghostzones, cctk_lsh - cctk_nghostzones) equal the SENR-layout
   kernels' values bit for bit (the kernels are the same arithmetic; see
-  tests/test_etk.py), and the right-hand sides agree with the C oracle to 1e-12;
+  tests/test_etk.py), except the outputs of the SENR group that builds its
+  x-neighbours by lane moves, whose reordered statements the compiler contracts
+  into FMAs differently: those agree to 1e-13 of each gridfunction's max norm;
+  and the right-hand sides agree with the C oracle to 1e-12;
 * every output point outside that range is still NaN (nothing written there),
   and no output is NaN inside it (no padding or out-of-range point was read);
 * det(gammabar) enforcement at every point equals the SENR kernel's bit for bit.
@@ -90,13 +93,27 @@ def test_plugin_form_equals_senr_kernels(torch_cuda, Nxx, ng, pad):
     k.rhss(gh, params, evol, rbar, rhs_e)
     k.constraints(gh, params, evol, aux)
     torch.cuda.synchronize()
+    # The SENR kernels of an output group that builds its x-neighbours by lane moves
+    # (bundle meta "xshift_kernels") order their statements differently, so the compiler
+    # contracts other multiply-adds into FMAs there: those outputs agree to rounding
+    # (1e-13 of each gridfunction's max norm), every other output bit for bit.
+    moved = [n for k, grp in enumerate(b.meta["rhs_groups"])
+             if "rhs_eval_g%d" % k in b.meta.get("xshift_kernels", []) for n in grp]
     for name, got, want in (("RbarDD", rbar, ev.auxevol), ("rhs", rhs_e, rhs),
                             ("H, MU", aux, torch.cat([H[None], MU]))):
         inside, outside, inner = _split(got, lsh, ng)
         ref = want.cpu().numpy()[:, inner]
         assert not np.isnan(inside).any(), name
         assert np.isnan(outside).all(), name
-        assert np.array_equal(inside, ref), (name, np.abs(inside - ref).max())
+        if name != "rhs":
+            assert np.array_equal(inside, ref), (name, np.abs(inside - ref).max())
+            continue
+        for q, gf in enumerate(b.evol):
+            if gf in moved:
+                e = np.abs(inside[q] - ref[q]).max() / max(np.abs(ref[q]).max(), 1e-300)
+                assert e <= 1e-13, (gf, e)
+            else:
+                assert np.array_equal(inside[q], ref[q]), (gf, np.abs(inside[q] - ref[q]).max())
     # right-hand sides against the oracle (its Ricci tensor, its RHS): 1e-12 of each
     # gridfunction's max norm, or, where the oracle's own FP64 rounding is that large (second
     # derivatives of the ~1 lapse and conformal factor, which the plugin kernels difference
