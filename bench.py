@@ -577,13 +577,14 @@ def main():
     t_bc = time_kernel(torch, lambda: ev.apply_bcs(ev.k_odd, composed=True), reps)
     t_detg = time_kernel(torch, lambda: ev.post_bcs(ev.k_odd), reps)
     # Dominant kernel: the slowest output group of the fused RHS + RK4 update (substep 1).
-    # Algorithmic traffic per interior point: every evolved gridfunction and the six RbarDD
-    # read once, and per output gridfunction y_n and the running total read, the stage
-    # output and the running total written.
+    # Algorithmic traffic per interior point: every gridfunction the kernel reads (evolved,
+    # and RbarDD where the group needs it; bundle meta "rhs_group_inputs") read once, and per
+    # output gridfunction y_n and the running total read, the stage output and the running
+    # total written.
     kdom = max(range(len(groups)), key=lambda k: t_group[k])
     n_out0 = len(groups[kdom])
     dom_flops = cost["rhs_eval"]["groups"][kdom]["flops"] + 6 * n_out0
-    dom_bytes = 8 * (ngf + 6 + 4 * n_out0)
+    dom_bytes = 8 * (b.meta["rhs_group_inputs"][kdom] + 4 * n_out0)
     t_dom = t_group[kdom]
     dom_name = "rhs_eval_rk4_substep_g%d" % kdom
     dom_gbs = dom_bytes * npts / t_dom / 1e9

@@ -20,6 +20,7 @@ Start-to-Finish drivers do.
 Each bundle is generated in a fresh interpreter (the registries are global):
 ``python -m nrpytutorial_amd.codegen.systems <bundle>``.
 """
+import re
 import sys
 import time
 import sympy as sp
@@ -330,6 +331,13 @@ ROW_TILE_OUTPUTS = {"aDD00", "aDD01", "aDD02", "aDD11", "aDD12", "aDD22", "trK"}
 XSHIFT_OUTPUTS = {"hDD00", "hDD01", "hDD02", "hDD11", "hDD12", "hDD22", "alpha", "cf"}
 
 
+def kernel_inputs(kernel_text):
+    """Distinct (array, gridfunction) pairs a generated kernel reads: its stencil reads and
+    the gridfunctions it copies into LDS."""
+    pat = r"\b(in_gfs|auxevol_gfs)(?:\[IDX4\(| \+ \(size_t\)|\[\(size_t\))(\w+)GF"
+    return set(re.findall(pat, kernel_text))
+
+
 def lds_tile_for_group(grp):
     """The LDS tile for the kernel of an output group, or None for the default."""
     return ROW_TILE if set(grp) == ROW_TILE_OUTPUTS else None
@@ -638,7 +646,7 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
                   "const REAL *restrict auxevol_gfs,const REAL *restrict in_gfs,REAL *restrict rhs_gfs")
     args_decl = ("const paramstruct params, const rfm_struct rfmstruct, const double *__restrict__ auxevol_gfs, "
                  "const double *__restrict__ in_gfs")
-    rhs_src, fused, group_costs = "", "", []
+    rhs_src, fused, group_costs, group_inputs = "", "", [], []
     # Large Cartesian grids: the RHS, Ricci and Hamiltonian kernels read their
     # mixed-derivative stencils from LDS tiles (codegen/lds.py). The small
     # curvilinear grids of the Brill-Lindquist runs (two points in phi) keep the
@@ -663,8 +671,10 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
         rhs_src += oc.outCfunction(outfile="returnstring", desc="BSSN RHSs of %s" % ", ".join(grp),
                                    name="rhs_eval_g%d" % k, params=rhs_params, body=body,
                                    loopopts="InteriorPoints,Enable_rfm_precompute")
-        fused += bd.fused_rk4_kernel("rhs_eval_rk4_substep_g%d" % k, args_decl, "", body,
-                                     loopopts="InteriorPoints,Enable_rfm_precompute")
+        kern = bd.fused_rk4_kernel("rhs_eval_rk4_substep_g%d" % k, args_decl, "", body,
+                                   loopopts="InteriorPoints,Enable_rfm_precompute")
+        group_inputs.append(len(kernel_inputs(kern)))
+        fused += kern
         fused += _fused_launcher("rhs_eval_rk4_substep_g%d" % k, "const rfm_struct *rfmstruct, "
                                  "const paramstruct *params, const double *auxevol_gfs, const double *in_gfs",
                                  "*params, *rfmstruct, auxevol_gfs, in_gfs", "InteriorPoints")
@@ -782,6 +792,9 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
              bt.instance_struct(), batch_table, '#include "../nrpy_batch.h"', batch_src,
              _mol_batch_glue(len(ric_groups), len(groups))]
     meta["rhs_groups"] = groups
+    # distinct gridfunctions each group's kernel reads (evolved and RbarDD): its algorithmic
+    # traffic is 8 (inputs + 4 outputs) bytes per point
+    meta["rhs_group_inputs"] = group_inputs
     meta["ricci_groups"] = ric_groups
     meta["vjp_slots"] = [sl[0] for sl in vslots]
     meta["ricci_vjp_slots"] = [sl[0] for sl in rslots]
