@@ -3,7 +3,11 @@
 * Cartesian reference metric, seeded flat + Gaussian state (the 192^3 RHS
   benchmark input at a size the oracle finishes in seconds): Ricci_eval,
   rhs_eval, Hamiltonian_constraint and momentum_constraint agree to 1e-12 of
-  each gridfunction's max norm (interior points).
+  each gridfunction's max norm (interior points); on grids fine enough that the FP64
+  oracle's own rounding of the lapse and conformal-factor second derivatives exceeds
+  that, rhs_eval and H are held to 2x the oracle's distance from the oracle built in
+  long double. The sizes include grids wider than one 64-lane row (the lane-move kernels'
+  overlapping last tile) and the narrowest grid they accept.
 * Spherical, symmetry axis 2, Brill-Lindquist head-on data (m1=m2=0.5 at
   z=+-0.5, RMAX=7.5): initial data at all points, boundary conditions +
   det(gammabar) enforcement, and an RK4 evolution whose state and
@@ -15,7 +19,7 @@ the rfm tables.
 import numpy as np
 import pytest
 
-from helpers import rel_maxnorm_err
+from helpers import rel_maxnorm_err, closer_to_exact
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -46,9 +50,14 @@ def _cart(torch, Nxx, seed=7):
 # tiles in x and z; (13,9,7) has an odd row length (the copy through registers) and
 # partial tiles in every direction.
 CART_SIZES = [(14, 12, 10), (13, 9, 7)]
+# The hDD,alpha,cf kernels build x-neighbours by lane moves over 64-lane rows: a grid wider
+# than one row and not a multiple of 64 moves its last row tile left to overlap the one
+# before (70: 58 lanes recompute points they do not store; 65: 63 lanes), and the narrowest
+# grid they accept has 6 points along x (a single partial row).
+X_TILE_SIZES = [(70, 9, 8), (65, 6, 7), (6, 7, 9)]
 
 
-@pytest.mark.parametrize("Nxx", CART_SIZES)
+@pytest.mark.parametrize("Nxx", CART_SIZES + X_TILE_SIZES)
 def test_cartesian_ricci_rhs_hamiltonian(torch_cuda, Nxx):
     torch = torch_cuda
     from nrpytutorial_amd.runtime import ptr, check
@@ -69,10 +78,24 @@ def test_cartesian_ricci_rhs_hamiltonian(torch_cuda, Nxx):
     torch.cuda.synchronize()
     err_mu = rel_maxnorm_err(g.interior(MU.cpu().numpy()), og.interior(og.momentum(y)))
     print("Ricci %.2e rhs %.2e H %.2e MU %.2e" % (err_r, err_rhs, err_h, err_mu))
-    assert err_r < TOL and err_rhs < TOL and err_h < TOL and err_mu < TOL
+    assert err_r < TOL and err_mu < TOL
+    if err_rhs >= TOL or err_h >= TOL:
+        # finer grids (dx = 0.29 along x at 70 points): the second derivatives of the unit
+        # lapse and conformal factor cancel their unit part, and the FP64 oracle's own
+        # rounding exceeds 1e-12 there; the GPU must then be as close to the exact value of
+        # the formulas (the oracle in long double) as 2x the FP64 oracle is
+        from oracle_lib import Oracle, OracleGrid
+        ogx = OracleGrid(Oracle("bssn_cartesian_ld"), Nxx, {})
+        ric_x = ogx.ricci(y)
+        for name, got, ref, exact in (
+                ("rhs", g.interior(rhs.cpu().numpy()), og.interior(rhs_ref), ogx.interior(ogx.rhs_eval(y, ric_x))),
+                ("H", g.interior(H.cpu().numpy())[None], og.interior(H_ref)[None], ogx.interior(ogx.hamiltonian(y))[None])):
+            ok, e_got, e_ref = closer_to_exact(got, ref, exact)
+            print("%s vs extended precision: GPU %.2e, oracle %.2e" % (name, e_got.max(), e_ref.max()))
+            assert ok, (name, e_got, e_ref)
 
 
-@pytest.mark.parametrize("Nxx", [(12, 12, 12), (13, 9, 7)])
+@pytest.mark.parametrize("Nxx", [(12, 12, 12), (13, 9, 7)] + X_TILE_SIZES)
 def test_cartesian_fused_substep_equals_rhs_plus_update(torch_cuda, Nxx):
     """rhs_eval_rk4_substep(S=0) == RK4 update applied to rhs_eval, at interior points."""
     torch = torch_cuda
@@ -90,6 +113,25 @@ def test_cartesian_fused_substep_equals_rhs_plus_update(torch_cuda, Nxx):
     exp_rt = g.interior(rhs * dt * (1.0 / 6.0))
     assert rel_maxnorm_err(g.interior(ev.k_odd).cpu().numpy(), exp_k.cpu().numpy()) < 1e-14
     assert rel_maxnorm_err(g.interior(ev.rt).cpu().numpy(), exp_rt.cpu().numpy()) < 1e-14
+
+
+def test_cartesian_rejects_grids_narrower_than_the_lane_move_halo(torch_cuda):
+    """Nxx0 = 5 < 2 x 3: every Cartesian BSSN entry point returns NRPY_ERR_SHAPE from its
+    host-side check, before any launch."""
+    import ctypes
+    from nrpytutorial_amd.runtime import Bundle
+    print("case: bssn_cartesian Nxx0=5 rejected", flush=True)
+    b = Bundle("bssn_cartesian")
+    G = b.NGHOSTS
+    # a consistent paramstruct whose only fault is its width (the 6-point grid above passes)
+    p = b.new_params(Nxx0=5, Nxx1=8, Nxx2=8, Nxx_plus_2NGHOSTS0=5 + 2 * G, Nxx_plus_2NGHOSTS1=8 + 2 * G,
+                     Nxx_plus_2NGHOSTS2=8 + 2 * G)
+    dummy = ctypes.c_void_p(0x1000)     # never dereferenced: the check returns first
+    for name in ("rhs_eval", "Ricci_eval"):
+        f = getattr(b.lib, name)
+        args = (None, ctypes.byref(p), dummy, dummy, dummy, None) if name == "rhs_eval" else \
+            (None, ctypes.byref(p), dummy, dummy, None)
+        assert f(*args) == -2, name
 
 
 BL = dict(BH1_mass=0.5, BH2_mass=0.5, BH1_posn_z=0.5, BH2_posn_z=-0.5, BH1_posn_x=0.0, BH1_posn_y=0.0,

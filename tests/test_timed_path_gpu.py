@@ -8,7 +8,9 @@ The oracle runs the reference's step (MoLtimestepping/C_Code_Generation.py:266-3
 Ricci, RHS, RK4 update loop, the layer-by-layer ghost fill, det(gammabar).
 
 * (14,12,10): even rows (direct global->LDS copies), partial tiles in x and z;
-  (13,9,7): odd rows (the copy through registers), partial tiles everywhere.
+  (13,9,7): odd rows (the copy through registers), partial tiles everywhere;
+  (70,9,8): wider than one 64-lane row, so the lane-move kernels (hDD,alpha,cf group)
+  run their last row tile moved left over the first, with the in-place last substep.
   6 steps from the seeded flat + Gaussian state.
 * 192^3 with seed 1 and dt = 0.5 dx: the benchmark's own state and step, 1 step.
 
@@ -52,7 +54,7 @@ def _run(torch, Nxx, seed, nsteps):
     return got, y, l2, l2_ref, ghost_gain(b, Nxx)
 
 
-@pytest.mark.parametrize("Nxx", [(14, 12, 10), (13, 9, 7)])
+@pytest.mark.parametrize("Nxx", [(14, 12, 10), (13, 9, 7), (70, 9, 8)])
 def test_graph_rk4_steps_match_oracle(torch_cuda, Nxx):
     print("case: bssn_cartesian MoL_graph 6 RK4 steps vs oracle, Nxx=%s, seed 7" % (Nxx,), flush=True)
     got, ref, l2, l2_ref, gain = _run(torch_cuda, Nxx, 7, 6)
