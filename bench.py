@@ -312,7 +312,9 @@ def bench_next_rows(torch, reps=10):
     """The rows beyond the BSSN hot path, each on one GPU:
     * Sommerfeld outer boundary (the Maxwell Start-to-Finish choice): vacuum Maxwell 192^3 on
       [-4,4]^3, apply_bcs_sommerfeld on the RHS (HIP-event average) and a whole RK4 step of
-      MoL_step_forward_in_time_sommerfeld (RHS + condition per substep, update at all points);
+      MoL_step_forward_in_time_sommerfeld (fused: interior RHS + update, then the condition
+      with the update at the outer points), of the unfused step in the reference's order,
+      and of the extrapolation-condition step (graph replay) for comparison;
     * curvilinear scalar wave, RK4 steps replayed from a hipGraph: SinhSphericalv2
       128 x 64 x 128 and SymTP 128 x 64 x 128 (2-D precomputed reference-metric tables).
     Rates are RK4 gridpoint-updates/s (4 RHS evaluations per interior point per step)."""
@@ -326,8 +328,14 @@ def bench_next_rows(torch, reps=10):
     dt = g.find_timestep(0.5)
     t_bc = time_kernel(torch, lambda: ev.apply_bcs_sommerfeld(ev.y_n, ev.k_even), reps)
     t_step = time_kernel(torch, lambda: ev.step_sommerfeld(dt, 1), reps)
+    t_unfused = time_kernel(torch, lambda: ev.step_sommerfeld(dt, 1, unfused=True), reps)
+    ev.initial_data()
+    t_extrap = time_kernel(torch, lambda: ev.step(dt, 1), reps)
     out["sommerfeld_maxwell_192"] = {"apply_bcs_sommerfeld_ms": 1e3 * t_bc, "rk4_step_ms": 1e3 * t_step,
-                                     "gp_per_s": 4.0 * N ** 3 / t_step}
+                                     "unfused_rk4_step_ms": 1e3 * t_unfused,
+                                     "extrapolation_rk4_step_ms": 1e3 * t_extrap,
+                                     "over_extrapolation": t_step / t_extrap, "gp_per_s": 4.0 * N ** 3 / t_step}
+    ev.free_graph()
     del ev, g
     for name, Nxx, pars in (("scalarwave_sinhsphericalv2", (128, 64, 128), dict(AMPL=10.0)),
                             ("scalarwave_symtp", (128, 64, 128), dict(AMAX=10.0))):
@@ -630,8 +638,11 @@ def main():
     if not a.no_next:
         nxt = bench_next_rows(torch)
         som = nxt["sommerfeld_maxwell_192"]
-        log("Sommerfeld Maxwell 192^3: apply_bcs_sommerfeld %.3f ms, RK4 step %.3f ms (%.3e gp/s); curvilinear "
+        log("Sommerfeld Maxwell 192^3: apply_bcs_sommerfeld %.3f ms, RK4 step %.3f ms (%.3e gp/s; unfused %.3f ms; "
+            "%.2fx the extrapolation step's %.3f ms); curvilinear "
             "scalar wave RK4 steps: %s" % (som["apply_bcs_sommerfeld_ms"], som["rk4_step_ms"], som["gp_per_s"],
+                                          som["unfused_rk4_step_ms"], som["over_extrapolation"],
+                                          som["extrapolation_rk4_step_ms"],
                                           ", ".join("%s %s %.3f ms (%.3e gp/s)" % (k, "x".join(map(str, v["grid"])),
                                                                                 v["rk4_step_ms"], v["gp_per_s"])
                                                     for k, v in nxt.items() if "grid" in v)))

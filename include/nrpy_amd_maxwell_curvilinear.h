@@ -32,12 +32,22 @@ int initial_data(const paramstruct *params, double *xx[3], double *in_gfs, hipSt
 int apply_bcs_sommerfeld(const paramstruct *params, double *xx[3], const bc_struct *bc, const int NUM_GFS,
                          const int8_t *gfs_parity, double *gfs, double *rhs_gfs, hipStream_t stream);
 
-/* One RK4 step of the driver's BoundaryCondition = "Sommerfeld" branch: per substep rhs_eval +
- * apply_bcs_sommerfeld, then the update at every point. */
+/* One RK4 step of the Start-to-Finish drivers' BoundaryCondition = "Sommerfeld" branch
+ * (per substep rhs_eval + apply_bcs_sommerfeld, then the RK4 update at every point; no ghost
+ * fill of the state). This bundle's grids have inner boundary points, so the step is
+ * MoL_step_forward_in_time_sommerfeld_unfused. xx: device coordinate arrays. */
 int MoL_step_forward_in_time_sommerfeld(const paramstruct *params, const rfm_struct *rfmstruct, double *xx[3],
                                         const bc_struct *bcstruct, double *y_n_gfs, double *k_odd_gfs,
                                         double *k_even_gfs, double *y_nplus1_running_total_gfs, const double dt,
                                         hipStream_t stream);
+
+/* The same step in the reference's order of operations: per substep rhs_eval +
+ * apply_bcs_sommerfeld, then the RK4 update at every point. */
+int MoL_step_forward_in_time_sommerfeld_unfused(const paramstruct *params, const rfm_struct *rfmstruct, double *xx[3],
+                                                const bc_struct *bcstruct, double *y_n_gfs, double *k_odd_gfs,
+                                                double *k_even_gfs, double *y_nplus1_running_total_gfs, const double dt,
+                                                hipStream_t stream);
+
 
 /* The Sommerfeld step above captured into a hipGraph (no counterpart in the reference; pointers, xx
  * and dt fixed at capture), replayed nsteps times per MoL_graph_launch, freed with MoL_graph_free. */

@@ -46,13 +46,42 @@ int apply_bcs_sommerfeld(const paramstruct *params, double *xx[3], const bc_stru
                          const int8_t *gfs_parity, double *gfs, double *rhs_gfs, hipStream_t stream);
 
 /* One RK4 step of the Start-to-Finish drivers' BoundaryCondition = "Sommerfeld" branch
- * (Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Cartesian.ipynb:240-246):
- * per substep rhs_eval + apply_bcs_sommerfeld, then the RK4 update at every point; no
- * ghost fill of the state. xx: device coordinate arrays. */
+ * (Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Cartesian.ipynb:240-246).
+ * Fused where the grid has no inner boundary points (this Cartesian bundle): per substep the
+ * interior RHS with its RK4 update (rhs_eval_rk4_substep_shell), then the radiation condition
+ * at the outer points with theirs (apply_bcs_sommerfeld_rk4). The values are those of the
+ * reference's order, MoL_step_forward_in_time_sommerfeld_unfused below. The first call
+ * allocates a right-hand-side scratch array (all gridfunctions, whole grid) kept in bcstruct;
+ * a step recorded into a graph before that array exists takes the unfused form.
+ * xx: device coordinate arrays. */
 int MoL_step_forward_in_time_sommerfeld(const paramstruct *params, const rfm_struct *rfmstruct, double *xx[3],
                                         const bc_struct *bcstruct, double *y_n_gfs, double *k_odd_gfs,
                                         double *k_even_gfs, double *y_nplus1_running_total_gfs, const double dt,
                                         hipStream_t stream);
+
+/* The same step in the reference's order of operations: per substep rhs_eval +
+ * apply_bcs_sommerfeld, then the RK4 update at every point. */
+int MoL_step_forward_in_time_sommerfeld_unfused(const paramstruct *params, const rfm_struct *rfmstruct, double *xx[3],
+                                                const bc_struct *bcstruct, double *y_n_gfs, double *k_odd_gfs,
+                                                double *k_even_gfs, double *y_nplus1_running_total_gfs, const double dt,
+                                                hipStream_t stream);
+
+/* Substep S (0-3) of the fused Sommerfeld step at the interior: the RHS and its RK4 update
+ * (as rhs_eval_rk4_substep), and at the interior points next to the boundary the RHS also
+ * into shell_rhs (NUM_EVOL_GFS gridfunctions, whole grid). No counterpart in the reference. */
+int rhs_eval_rk4_substep_shell(const int S, const paramstruct *params, const double *in_gfs, double *y_n,
+                               double *rt, double *out, double *shell_rhs, const double dt,
+                               hipStream_t stream);
+
+/* Substep S of the fused Sommerfeld step at the outer points: the radiation condition
+ * (apply_bcs_sommerfeld's arithmetic) with the RHS at each point's inward neighbour read from
+ * R and its own written there, and the RK4 update of that point (nrpy_rk4_update_all's
+ * arithmetic), layer by layer. gfs: the substep's input state. NRPY_ERR_PARAM for grids with
+ * inner boundary points. No counterpart in the reference. */
+int apply_bcs_sommerfeld_rk4(const int S, const paramstruct *params, double *xx[3], const bc_struct *bc,
+                             const int NUM_GFS, double *gfs, double *R, double *y_n, double *rt,
+                             double *out, const double dt, hipStream_t stream);
+
 
 /* The Sommerfeld step above captured into a hipGraph (no counterpart in the reference; pointers, xx
  * and dt fixed at capture), replayed nsteps times per MoL_graph_launch, freed with MoL_graph_free. */

@@ -107,19 +107,53 @@ def _mol_glue(has_ricci, has_detg, rhs_args_decl, rhs_call_args):
         "  if (!g) return;\n  (void)hipGraphExecDestroy(g->exec);\n  (void)hipGraphDestroy(g->graph);\n  delete g;\n}\n")
 
 
-def _sommerfeld_glue(fd_order, rhs_call, **defaults):
+def _sommerfeld_glue(fd_order, rhs_call, shell_call=None, **defaults):
     """Sommerfeld radiation condition (CurviBoundaryConditions.py:259-658) and the RK4 step of
     the Start-to-Finish drivers' BoundaryCondition = "Sommerfeld" branch
     (Tutorial-Start_to_Finish-Solving_Maxwells_Equations_in_Vacuum-Cartesian.ipynb:240-246):
     RHS_string = rhs_eval + apply_bcs_sommerfeld on the RHS, no post-RHS boundary fill, the
-    update at every point. Returns (HIP source, per-gridfunction parameters for the meta)."""
+    update at every point, as MoL_step_forward_in_time_sommerfeld_unfused.
+
+    With ``shell_call`` (a call of the bundle's <fused kernel>_shell with IN, OUT, R and S
+    placeholders) MoL_step_forward_in_time_sommerfeld takes the fused form where it can: per
+    substep, the fused interior RHS + RK4 update (which also leaves the RHS at the interior
+    points next to the boundary in the scratch R), then the radiation condition at the outer
+    points fused with their update, layer by layer (apply_bcs_sommerfeld_rk4). The values are
+    those of the unfused step: the same RHS and update arithmetic at every point, only the
+    separate all-points update pass is gone. Grids with inner boundary points, and a step
+    recorded into a graph before R exists, take the unfused step. Returns (HIP source,
+    per-gridfunction parameters for the meta)."""
     from ..CurviBoundaryConditions import CurviBoundaryConditions as cbcs
     from ..MoLtimestepping import C_Code_Generation as MoL
     bcs = cbcs.sommerfeld_boundary_condition_class(fd_order=fd_order, **defaults)
     rhs_string = (rhs_call + "; apply_bcs_sommerfeld(params, xx, bcstruct, NUM_EVOL_GFS, evol_gf_parity, "
                   "RK_INPUT_GFS, RK_OUTPUT_GFS, stream);")
-    step = MoL.mol_step_function("MoL_step_forward_in_time_sommerfeld",
-                                 "const rfm_struct *rfmstruct, double *xx[3], const bc_struct *bcstruct", rhs_string)
+    sig = "const rfm_struct *rfmstruct, double *xx[3], const bc_struct *bcstruct"
+    step = MoL.mol_step_function("MoL_step_forward_in_time_sommerfeld_unfused", sig, rhs_string)
+    call_unfused = ("MoL_step_forward_in_time_sommerfeld_unfused(params, rfmstruct, xx, bcstruct, y_n_gfs, "
+                    "k_odd_gfs, k_even_gfs, y_nplus1_running_total_gfs, dt, stream)")
+    if shell_call is None:
+        fused_body = "  return %s;\n" % call_unfused
+    else:
+        subs = ""
+        for S, (i, o) in enumerate((("y_n_gfs", "k_odd_gfs"), ("k_odd_gfs", "k_even_gfs"),
+                                    ("k_even_gfs", "k_odd_gfs"), ("k_odd_gfs", "y_n_gfs"))):
+            subs += ("  /* substep %d: %s -> %s */\n" % (S, i, o) +
+                     "  if ((rc = %s)) return rc;\n" % shell_call.replace("IN", i).replace("OUT", o)
+                     .replace("SUBSTEP", str(S)) +
+                     "  if ((rc = apply_bcs_sommerfeld_rk4(%d, params, xx, bcstruct, NUM_EVOL_GFS, %s, R, y_n_gfs, "
+                     "y_nplus1_running_total_gfs, %s, dt, stream))) return rc;\n" % (S, i, o))
+        fused_body = ("  if (!params || !bcstruct || !xx || !y_n_gfs || !k_odd_gfs || !k_even_gfs ||\n"
+                      "      !y_nplus1_running_total_gfs) return NRPY_ERR_NULL;\n"
+                      "  bool inner = false;\n"
+                      "  for (int gz = 0; gz < NGHOSTS; gz++) inner = inner || bcstruct->num_ib_gz_pts[gz] > 0;\n"
+                      "  double *R = inner ? nullptr : nrpy_sommerfeld_scratch(bcstruct, NUM_EVOL_GFS, stream);\n"
+                      "  if (!R) return %s;\n  int rc;\n%s  return NRPY_OK;\n" % (call_unfused, subs))
+    step += ("// One RK4 step of the Sommerfeld branch (see MoL_step_forward_in_time_sommerfeld_unfused for\n"
+             "// the reference's order of operations; the fused form gives the same values).\n"
+             'extern "C" int MoL_step_forward_in_time_sommerfeld(const paramstruct *params, %s, double *y_n_gfs, '
+             "double *k_odd_gfs, double *k_even_gfs, double *y_nplus1_running_total_gfs, const double dt, "
+             "hipStream_t stream) {\n%s}\n" % (sig, fused_body))
     meta = {"fd_order": fd_order, "at_inf": list(bcs.vars_at_infinity.values()),
             "radial_falloff_power": list(bcs.vars_radial_falloff_power.values()),
             "speed": list(bcs.vars_speed.values())}
@@ -131,7 +165,12 @@ def _sommerfeld_glue(fd_order, rhs_call, **defaults):
         "double *y_nplus1_running_total_gfs, const double dt, MoL_graph **out) {\n"
         "  if (!out) return NRPY_ERR_NULL;\n  *out = nullptr;\n"
         "  hipStream_t s;\n"
-        "  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return NRPY_ERR_LAUNCH;\n"
+        "  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return NRPY_ERR_LAUNCH;\n" +
+        ("  if (bcstruct) {  /* the fused step's scratch, allocated before capture */\n"
+         "    bool inner = false;\n"
+         "    for (int gz = 0; gz < NGHOSTS; gz++) inner = inner || bcstruct->num_ib_gz_pts[gz] > 0;\n"
+         "    if (!inner) (void)nrpy_sommerfeld_scratch(bcstruct, NUM_EVOL_GFS, s);\n"
+         "  }\n" if shell_call is not None else "") +
         "  MoL_graph *g = new MoL_graph();\n"
         "  int rc = NRPY_OK;\n"
         "  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = NRPY_ERR_LAUNCH;\n"
@@ -288,6 +327,48 @@ def _fused_launcher(name, args_sig, call_args, loopopts):
             % (name, args_sig, block, n[0], n[1], n[2], cases))
 
 
+# the fused Sommerfeld step's interior substep in the Cartesian scalar-wave and Maxwell bundles
+SHELL_CALL = ("rhs_eval_rk4_substep_shell(SUBSTEP, params, IN, y_n_gfs, y_nplus1_running_total_gfs, OUT, R, dt, "
+              "stream)")
+
+
+def _fused_shell_kernel(fused_kernel, name):
+    """The fused RHS + RK4 kernel ``name`` (the text bd.fused_rk4_kernel wrote) as
+    ``<name>_shell``: the same computation, and at the interior points next to the grid's
+    boundary (one index at its first or last interior value) it also stores each RHS into
+    shell_rhs, where the fused Sommerfeld step's outer-point kernels read it."""
+    import re as _re
+    i = fused_kernel.index("%s__gpu(" % name)
+    start = fused_kernel.rindex("template <int S>", 0, i)
+    end = fused_kernel.index("\n}\n", i) + 3
+    k = fused_kernel[start:end].replace("%s__gpu(" % name, "%s_shell__gpu(" % name, 1)
+    k = k.replace("const double dt) {", "const double dt, double *__restrict__ shell_rhs) {", 1)
+    stores = _re.findall(r"NRPY_RK4_STORE\(S, \d+, __rhs_(\w+)\);", k)
+    last = [m.end() for m in _re.finditer(r"NRPY_RK4_STORE\(S, \d+, __rhs_\w+\);", k)][-1]
+    block = ("\n  if (i0 == NGHOSTS || i0 == Nxx0 + NGHOSTS - 1 || i1 == NGHOSTS || i1 == Nxx1 + NGHOSTS - 1 ||"
+             " i2 == NGHOSTS || i2 == Nxx2 + NGHOSTS - 1) {\n" +
+             "".join("    shell_rhs[IDX4(%sGF, i0, i1, i2)] = __rhs_%s;\n" % (g, g) for g in stores) + "  }")
+    return k[:last] + block + k[last:]
+
+
+def _fused_shell_launcher(name, args_sig, call_args):
+    """Launcher of <name>_shell, with the block shape of <name> (bd.lp.kernel_id)."""
+    kid = bd.lp.kernel_id(name)
+    cases = "".join(
+        "    case %d: hipLaunchKernelGGL(%s_shell__gpu<%d>, grid, block, 0, stream, %s, y_n, rt, out, dt, shell_rhs); "
+        "break;\n" % (s, name, s, call_args) for s in range(4))
+    return ('extern "C" int %s_shell(const int S, %s, double *y_n, double *rt, double *out, double *shell_rhs, '
+            "const double dt, hipStream_t stream) {\n"
+            "  if (!params || !y_n || !rt || !out || !shell_rhs) return NRPY_ERR_NULL;\n"
+            "  if (S < 0 || S > 3) return NRPY_ERR_PARAM;\n"
+            "  if (nrpy_check_params(params)) return NRPY_ERR_SHAPE;\n"
+            "  const dim3 block = nrpy_block(%d, %d, %d, %d);\n"
+            "  const dim3 grid(nrpy_nblocks(block, params->Nxx0, params->Nxx1, params->Nxx2));\n"
+            "  switch (S) {\n%s  }\n"
+            "  return hipGetLastError() == hipSuccess ? 0 : NRPY_ERR_LAUNCH;\n}\n"
+            % (name, args_sig, kid, bd.lp.BLOCK_X, bd.lp.BLOCK_Y, bd.lp.BLOCK_Z, cases))
+
+
 # Output groups of the BSSN RHS, one kernel each: fewer live values per kernel. Measured on
 # MI355X at 192^3 (fused RHS + RK4 update, one substep): one kernel 28 ms; two groups
 # (aDD,hDD,alpha,cf,trK | lambdaU,vetU,betU) 10.2 ms; these three 8.5 ms. Grouping
@@ -399,9 +480,12 @@ def scalarwave_cartesian():
     fused = bd.fused_rk4_kernel("rhs_eval_rk4_substep", args_decl, "", body, loopopts="InteriorPoints")
     fused += _fused_launcher("rhs_eval_rk4_substep", "const rfm_struct *rfmstruct, const paramstruct *params, "
                              "const double *auxevol_gfs, const double *in_gfs", "*params, in_gfs", "InteriorPoints")
+    fused += _fused_shell_kernel(fused, "rhs_eval_rk4_substep") + _fused_shell_launcher(
+        "rhs_eval_rk4_substep", "const paramstruct *params, const double *in_gfs", "*params, in_gfs")
     id_src = _initial_data_kernel([swid.uu_ID, swid.vv_ID], ["uu", "vv"])
     vjp_src, slots, vouts, vdep = adj.vjp_kernels("rhs_eval", exprs, "", "InteriorPoints", "")
-    som_src, som_meta = _sommerfeld_glue(4, "rhs_eval(params, RK_INPUT_GFS, RK_OUTPUT_GFS, stream)")
+    som_src, som_meta = _sommerfeld_glue(4, "rhs_eval(params, RK_INPUT_GFS, RK_OUTPUT_GFS, stream)",
+                                         shell_call=SHELL_CALL)
     pre, meta = _header(3)
     parts = [pre, bd.coord_host_functions("Cartesian"), '#include "../nrpy_runtime.h"', bd.rfm_struct_code(),
              rhs_src, fused, id_src, vjp_src,
@@ -449,10 +533,13 @@ def maxwell_cartesian():
     fused = bd.fused_rk4_kernel("rhs_eval_rk4_substep", args_decl, "", body, loopopts="InteriorPoints")
     fused += _fused_launcher("rhs_eval_rk4_substep", "const rfm_struct *rfmstruct, const paramstruct *params, "
                              "const double *auxevol_gfs, const double *in_gfs", "*params, in_gfs", "InteriorPoints")
+    fused += _fused_shell_kernel(fused, "rhs_eval_rk4_substep") + _fused_shell_launcher(
+        "rhs_eval_rk4_substep", "const paramstruct *params, const double *in_gfs", "*params, in_gfs")
     id_src = _initial_data_kernel(id_rhss, names)
     coords = bd.coord_host_functions("Cartesian")
     # the Start-to-Finish Maxwell run's choice: second-order radial derivative, falloff power 3
-    som_src, som_meta = _sommerfeld_glue(2, "rhs_eval(params, RK_INPUT_GFS, RK_OUTPUT_GFS, stream)")
+    som_src, som_meta = _sommerfeld_glue(2, "rhs_eval(params, RK_INPUT_GFS, RK_OUTPUT_GFS, stream)",
+                                         shell_call=SHELL_CALL)
     pre, meta = _header(3)
     vjp_src, slots, vouts, vdep = adj.vjp_kernels("rhs_eval", exprs, "", "InteriorPoints", "")
     parts = [pre, coords, '#include "../nrpy_runtime.h"', bd.rfm_struct_code(), rhs_src, con_src, fused, id_src,

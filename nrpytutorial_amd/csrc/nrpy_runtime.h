@@ -90,6 +90,9 @@ struct bc_struct {
   int *d_tnegmask = nullptr;
   std::vector<int> h_tsrc, h_trowptr, h_tghost, h_tnegmask;
   std::vector<double> h_tcoeff;
+  // right-hand-side scratch of the fused Sommerfeld RK4 step (nrpy_sommerfeld_scratch)
+  double *d_som_rhs = nullptr;
+  size_t som_rhs_n = 0;
 };
 
 struct nrpy_gf_parity_args {
@@ -353,7 +356,28 @@ extern "C" void bcstruct_free(bc_struct *bc) {
   (void)hipFree(bc->d_tghost);
   (void)hipFree(bc->d_tcoeff);
   (void)hipFree(bc->d_tnegmask);
+  (void)hipFree(bc->d_som_rhs);
   delete bc;
+}
+
+// Right-hand-side scratch of the fused Sommerfeld RK4 step: num_gfs gridfunctions over the
+// whole grid, owned by the boundary struct (a grid's struct always asks for the same size,
+// so it is allocated once and freed by bcstruct_free). The first call allocates it, which
+// is not possible while the stream is being captured: then, and if the allocation fails,
+// the result is nullptr and the caller takes the unfused step, which needs no scratch.
+static double *nrpy_sommerfeld_scratch(const bc_struct *bc, const int num_gfs, hipStream_t stream) {
+  bc_struct *b = const_cast<bc_struct *>(bc);
+  const size_t n = (size_t)num_gfs * bc->N[0] * bc->N[1] * bc->N[2];
+  if (b->d_som_rhs && b->som_rhs_n >= n) return b->d_som_rhs;
+  if (b->d_som_rhs) return nullptr;              // never resized: a captured step may hold it
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+  if (hipMalloc((void **)&b->d_som_rhs, n * sizeof(double)) != hipSuccess) {
+    b->d_som_rhs = nullptr;
+    return nullptr;
+  }
+  b->som_rhs_n = n;
+  return b->d_som_rhs;
 }
 
 // driver_bcstruct.h:1-44 as one call: maps at all points, then bcstruct, then

@@ -80,6 +80,7 @@ PROTOTYPES = {
 SOMMERFELD_PROTOTYPES = {
     "apply_bcs_sommerfeld": (_I, [_VP, _DPP, _VP, _I, _VP, _VP, _VP, _VP]),
     "MoL_step_forward_in_time_sommerfeld": (_I, [_VP, _VP, _DPP, _VP, _VP, _VP, _VP, _VP, _D, _VP]),
+    "MoL_step_forward_in_time_sommerfeld_unfused": (_I, [_VP, _VP, _DPP, _VP, _VP, _VP, _VP, _VP, _D, _VP]),
     "MoL_sommerfeld_graph_create": (_I, [_VP, _VP, _DPP, _VP, _VP, _VP, _VP, _VP, _D, ctypes.POINTER(_VP)]),
 }
 KERNEL_PROTOTYPES = {
@@ -339,9 +340,17 @@ class Evolution:
                                             self.g.bundle.evol_parity, ptr(gfs), ptr(rhs_gfs), self.s()),
               "apply_bcs_sommerfeld")
 
-    def step_sommerfeld(self, dt, nsteps=1, use_graph=True):
+    def step_sommerfeld(self, dt, nsteps=1, use_graph=True, unfused=False):
         """nsteps RK4 steps with the Sommerfeld condition on the RHS and the update at every point,
-        replayed from a captured hipGraph (or launched eagerly)."""
+        replayed from a captured hipGraph (or launched eagerly). unfused: launch the step in the
+        reference's order of operations (MoL_step_forward_in_time_sommerfeld_unfused) eagerly;
+        the default step is fused where the grid allows it and gives the same values."""
+        if unfused:
+            for _ in range(nsteps):
+                check(self.lib.MoL_step_forward_in_time_sommerfeld_unfused(
+                    self.g.pp(), self.g.rfm, self.g.xx_dev_ptrs, self.g.bc, ptr(self.y_n), ptr(self.k_odd),
+                    ptr(self.k_even), ptr(self.rt), float(dt), self.s()), "MoL_step_forward_in_time_sommerfeld_unfused")
+            return
         if use_graph:
             if self.som_graph is None or self.som_graph_dt != dt:
                 self.free_graph()

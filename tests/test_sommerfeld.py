@@ -3,7 +3,8 @@
 * The Python interface reproduces the dictionaries printed by
   Tutorial-SommerfeldBoundaryCondition.ipynb (cells 30-36).
 * The product's d x^i/d r contraction reduces to the closed forms of the
-  Cartesian, Spherical and Cylindrical reference metrics.
+  Cartesian, Spherical and Cylindrical reference metrics, and in Cartesian
+  coordinates stays regular on the z axis (a grid point when Nxx0 and Nxx1 are odd).
 * The oracle's apply_bcs_sommerfeld, pinned by what the mathematics fixes:
   - an exact outgoing spherical wave f = f_inf + g(r - v t)/r satisfies
     d_t f = -v [d_r f + (f - f_inf)/r], so with falloff power 0 the ghost-point
@@ -88,6 +89,12 @@ for cs in ("Cartesian", "Spherical", "Cylindrical"):
         ra, rb = complex(sp.N(sp.sympify(r).subs(p))), complex(sp.N(sp.sympify(want_r).subs(p)))
         assert abs(a - b) < 1e-12 * max(1.0, abs(b)), (cs, a, b)
         assert abs(ra - rb) < 1e-12 * max(1.0, abs(rb)), (cs, ra, rb)
+    if cs == "Cartesian":
+        # on the z axis (a grid point when Nxx0 and Nxx1 are odd) the form is regular:
+        # d f/d r = sign(z) d f/d z, where the spherical Jacobian's cofactors give 0/0
+        for z in (0.7, -1.3):
+            a = complex(sp.N(sp.sympify(c).subs({x[0]: 0, x[1]: 0, x[2]: z}).subs({fd[0]: 0.31, fd[1]: -1.7, fd[2]: 0.9})))
+            assert abs(a - (0.9 if z > 0 else -0.9)) < 1e-12, (z, a)
 print("ok")
 """
 
