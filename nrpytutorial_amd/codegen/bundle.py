@@ -242,14 +242,17 @@ def rfm_struct_code():
 
 
 def check_params_code():
-    from . import lds
+    from . import lds, addressing
     # kernels whose x-neighbours come from lane moves need rows of at least 2 HX points
     nx_min = max(1, lds.MIN_NXX0)
+    # kernels with tile-relative addressing need their 32-bit lane offsets below 2^32 bytes
+    tile = addressing.bound_check_code()
     return ("static int nrpy_check_params(const paramstruct *p) {\n"
             "  if (p->Nxx0 < %d || p->Nxx1 < 1 || p->Nxx2 < 1) return 1;\n" % nx_min +
             "  if (p->Nxx_plus_2NGHOSTS0 != p->Nxx0 + 2*NGHOSTS || p->Nxx_plus_2NGHOSTS1 != p->Nxx1 + 2*NGHOSTS ||\n"
             "      p->Nxx_plus_2NGHOSTS2 != p->Nxx2 + 2*NGHOSTS) return 1;\n"
-            "  if ((double)p->Nxx_plus_2NGHOSTS0 * p->Nxx_plus_2NGHOSTS1 * p->Nxx_plus_2NGHOSTS2 > 2147483647.0) return 1;\n"
+            "  if ((double)p->Nxx_plus_2NGHOSTS0 * p->Nxx_plus_2NGHOSTS1 * p->Nxx_plus_2NGHOSTS2 > 2147483647.0) return 1;\n" +
+            ("  if (%s) return 1;\n" % tile if tile else "") +
             "  return 0;\n}\n")
 
 

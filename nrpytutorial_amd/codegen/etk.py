@@ -35,6 +35,7 @@ from .. import loop as lp
 from ..outputC import lhrh
 from . import bundle as bd
 from . import lds
+from . import addressing
 
 FD_ORDER = 4
 
@@ -85,12 +86,24 @@ static int nrpy_etk_geometry(const nrpy_cctkGH *g, const int interior, nrpy_etk_
   if (total >= ((size_t)1 << 31)) return NRPY_ERR_SHAPE;
   out->ash0 = g->cctk_ash[0];
   out->ash1 = g->cctk_ash[1];
+  /* kernels with tile-relative addressing: 32-bit lane offsets within a plane */
+  if (interior && NRPY_ETK_TILE_TOO_WIDE((size_t)out->ash0, (size_t)out->ash1)) return NRPY_ERR_SHAPE;
   out->invdx0 = 1.0 / g->cctk_delta_space[0];
   out->invdx1 = 1.0 / g->cctk_delta_space[1];
   out->invdx2 = 1.0 / g->cctk_delta_space[2];
   return NRPY_OK;
 }
 """
+
+# Tile-relative addressing (codegen/addressing.py) of the RHS group kernels, by the
+# group's outputs. Measured at 192^3, one RHS evaluation (the three groups): 5.27 ms
+# without; 5.03 with the aDD,trK group in form "xc"; 4.55 with the hDD,alpha,cf group in
+# "xc" as well (it builds no x-neighbours by lane moves here). The Lambda-bar group is
+# left as it is: 4.63 ms with it in "xc", 5.38 in "xl".
+ETK_TILE_ADDRESS_OUTPUTS = (
+    ({"aDD00", "aDD01", "aDD02", "aDD11", "aDD12", "aDD22", "trK"}, "xc"),
+    ({"hDD00", "hDD01", "hDD02", "hDD11", "hDD12", "hDD22", "alpha", "cf"}, "xc"),
+)
 
 _ACCESS = re.compile(r"\b([A-Za-z]\w*?)(_rhs)?GF\[CCTK_GFINDEX3D\(cctkGH,")
 _WRITE = re.compile(r"\b(\w+GF)\[CCTK_GFINDEX3D\(cctkGH,[^\]]*\)\]\s*=(?!=)")
@@ -149,9 +162,9 @@ def etk_kernel(name, desc, body, table, interior=True):
     for d, t in ((0, "x"), (1, "y"), (2, "z")):
         head.append("const int i%d = nrpy_lo%d + nrpy_b%s*(int)blockDim.%s + (int)threadIdx.%s;" % (d, d, t, t, t))
     head.append("if (i0 >= nrpy_hi0 || i1 >= nrpy_hi1 || i2 >= nrpy_hi2) return;")
-    text = ("/* %s */\n__global__ void __launch_bounds__(%d) %s__gpu(const nrpy_etk_geom cctkGH, "
+    text = ("/* %s */\n__global__ void __launch_bounds__(%s) %s__gpu(const nrpy_etk_geom cctkGH, "
             "const paramstruct params, %s) {\n%s}\n"
-            % (desc, lp.MAX_THREADS, name, ", ".join(d for d, _ in ptrs),
+            % (desc, lp.launch_bounds(name), name, ", ".join(d for d, _ in ptrs),
                "".join("  " + l + "\n" for l in head + body.splitlines())))
     block = "dim3(%d, %d, %d)" % (lp.BLOCK_X, lp.BLOCK_Y, lp.BLOCK_Z)
     if interior and lp.LDS_STAGING:
@@ -160,6 +173,12 @@ def etk_kernel(name, desc, body, table, interior=True):
         if staged is not None:
             text, block = staged, "dim3(%d, %d, %d)" % tile
             lp.LDS_KERNELS[name] = list(tile)
+        else:
+            tile = (lp.BLOCK_X, lp.BLOCK_Y, lp.BLOCK_Z)     # the launch's fixed block
+        form = addressing.form_for(name)
+        if form:
+            # LDS_STAGING is the bundle's NGHOSTS (FD order / 2 + 1)
+            text = addressing.tile_address(text, form, tile, lp.LDS_STAGING, name, access="CCTK_GFINDEX3D")
     return text, (block, [slot for _, slot in ptrs])
 
 
@@ -195,7 +214,7 @@ def entry_point(name, desc, args, fills, kernels, interior=True):
 def bssn_etk(bundle_name="bssn_etk"):
     from ..BSSN import BSSN_quantities as Bq, BSSN_RHSs as rhs, BSSN_gauge_RHSs as gaugerhs
     from ..BSSN import Enforce_Detgammabar_Constraint as EGC, BSSN_constraints as bssncon
-    from .systems import rhs_groups, ricci_groups, lds_tile_for_group
+    from .systems import rhs_groups, ricci_groups, lds_tile_for_group, MIN_TWO_WAVES_OUTPUTS
     t0 = time.time()
     par.set_parval_from_str("grid::DIM", 3)
     par.set_parval_from_str("finite_difference::FD_CENTDERIVS_ORDER", FD_ORDER)
@@ -265,6 +284,11 @@ def bssn_etk(bundle_name="bssn_etk"):
     for k, grp in enumerate(rhs_groups([n for n, _ in pairs])):
         if lds_tile_for_group(grp):
             lp.LDS_TILES["Baikal_BSSN_RHSs_g%d" % k] = lds_tile_for_group(grp)
+        for outs, form in ETK_TILE_ADDRESS_OUTPUTS:
+            if set(grp) == outs:
+                addressing.TILE_ADDRESSING["Baikal_BSSN_RHSs_g%d" % k] = form
+        if set(grp) == MIN_TWO_WAVES_OUTPUTS:
+            lp.MIN_WAVES_PER_EU["Baikal_BSSN_RHSs_g%d" % k] = 2
         sel = [e for (n, _), e in zip(pairs, evol_rhss) if n in grp]
         body = fin.FD_outputC("returnstring", sel, params="outCverbose=False", upwindcontrolvec=betaU)
         flops += bd.count_body(body)["flops"]
@@ -333,14 +357,17 @@ def bssn_etk(bundle_name="bssn_etk"):
         protos.append('extern "C" int %s(%s);' % (drv, sig))
 
     ps, members = bd.paramstruct_decl()
+    too_wide = addressing.bound_check_code("(A0)", "(A1)") or "0"
     pre = ('#include "../nrpy_runtime_pre.h"\n#define NGHOSTS %d\n#define NRPY_ETK_NPTRS %d\n'
-           % (FD_ORDER // 2 + 1, table.size) + ps + "\n" + GEOMETRY)
+           % (FD_ORDER // 2 + 1, table.size) + ps + "\n" +
+           "#define NRPY_ETK_TILE_TOO_WIDE(A0, A1) (%s)\n" % too_wide + GEOMETRY)
     meta = {"bundle": bundle_name, "family": "bssn_etk", "NGHOSTS": FD_ORDER // 2 + 1, "FD_order": FD_ORDER,
             "paramstruct": members,
             "gfs": {"evol": table.evol, "aux": table.aux, "auxevol": table.auxevol},
             "pointer_slots": {"evol": 0, "rhs": table.n_evol, "auxevol": 2 * table.n_evol,
                               "aux": 2 * table.n_evol + table.n_auxevol, "total": table.size},
             "lds_kernels": dict(lp.LDS_KERNELS), "cost": costs, "prototypes": protos,
+            "tile_addressing": dict(sorted(addressing.APPLIED.items())),
             "kernels": list(costs)}
     return bd.write_bundle(bundle_name, [pre] + src + [drivers], meta)
 

@@ -33,6 +33,7 @@ from .. import finite_difference as fin
 from .. import outputC as oc
 from ..outputC import lhrh
 from . import lds
+from . import addressing
 from . import bundle as bd
 from . import adjoint as adj
 from . import batch as bt
@@ -411,6 +412,22 @@ ROW_TILE_OUTPUTS = {"aDD00", "aDD01", "aDD02", "aDD11", "aDD12", "aDD22", "trK"}
 # or two waves per SIMD, were measured slower that way (2.34 -> 2.95, 1.15 -> 1.42 ms).
 XSHIFT_OUTPUTS = {"hDD00", "hDD01", "hDD02", "hDD11", "hDD12", "hDD22", "alpha", "cf"}
 
+# The aDD,trK group needs a few registers more than two waves per SIMD allow once its
+# lapse and conformal-factor stencils are differenced; asking for two waves fits it (no
+# spills) and keeps its occupancy.
+MIN_TWO_WAVES_OUTPUTS = {"aDD00", "aDD01", "aDD02", "aDD11", "aDD12", "aDD22", "trK"}
+
+# Tile-relative addressing (codegen/addressing.py) of the group kernels, chosen by the
+# group's outputs. Measured at 192^3, fused RK4 substep: the aDD,trK group 1.54 -> 1.44 ms
+# with "xc"; the Lambda-bar group 2.34 -> 2.15 ms with "xl", but 2.66 ms with "xc" (at one
+# wave per SIMD the fused 16-byte loads of x-neighbours cost it registers). Not used: the
+# lane-move group with "xl" (1.03 -> 1.44 ms: its halo loads take per-lane offsets) and the
+# Ricci kernels (1.16 ms either way).
+TILE_ADDRESS_OUTPUTS = (
+    ({"aDD00", "aDD01", "aDD02", "aDD11", "aDD12", "aDD22", "trK"}, "xc"),
+    ({"betU0", "betU1", "betU2", "lambdaU0", "lambdaU1", "lambdaU2", "vetU0", "vetU1", "vetU2"}, "xl"),
+)
+
 
 def kernel_inputs(kernel_text):
     """Distinct (array, gridfunction) pairs a generated kernel reads: its stencil reads and
@@ -741,14 +758,18 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     bd.lp.LDS_STAGING = (FD_order // 2 + 1) if CoordSystem == "Cartesian" else 0
     lds.MIN_NXX0 = 0
     lds.XSHIFT_KERNELS = set()
-    if CoordSystem == "Cartesian":
-        # the aDD,trK group needs a few registers more than two waves per SIMD allow once its
-        # lapse and conformal-factor stencils are differenced; asking for two waves fits it
-        # (no spills) and keeps its occupancy
-        bd.lp.MIN_WAVES_PER_EU["rhs_eval_rk4_substep_g0"] = bd.lp.MIN_WAVES_PER_EU["rhs_eval_g0"] = 2
+    addressing.TILE_ADDRESSING = {}
+    addressing.TILE_BOUND[:] = [0, 0, 0]
+    addressing.APPLIED.clear()
     for k, grp in enumerate(groups):
+        if CoordSystem == "Cartesian" and set(grp) == MIN_TWO_WAVES_OUTPUTS:
+            bd.lp.MIN_WAVES_PER_EU["rhs_eval_rk4_substep_g%d" % k] = bd.lp.MIN_WAVES_PER_EU["rhs_eval_g%d" % k] = 2
         if bd.lp.LDS_STAGING and set(grp) == XSHIFT_OUTPUTS:
             lds.XSHIFT_KERNELS |= {"rhs_eval_g%d" % k, "rhs_eval_rk4_substep_g%d" % k}
+        for outs, form in TILE_ADDRESS_OUTPUTS if bd.lp.LDS_STAGING else ():
+            if set(grp) == outs:
+                addressing.TILE_ADDRESSING["rhs_eval_g%d" % k] = form
+                addressing.TILE_ADDRESSING["rhs_eval_rk4_substep_g%d" % k] = form
         if bd.lp.LDS_STAGING and lds_tile_for_group(grp):
             bd.lp.LDS_TILES["rhs_eval_g%d" % k] = bd.lp.LDS_TILES["rhs_eval_rk4_substep_g%d" % k] = \
                 lds_tile_for_group(grp)
@@ -890,6 +911,7 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     meta["kernel_ids"] = dict(bd.lp.KERNEL_IDS)
     meta["lds_kernels"] = dict(bd.lp.LDS_KERNELS)
     meta["xshift_kernels"] = sorted(lds.XSHIFT_KERNELS)
+    meta["tile_addressing"] = dict(sorted(addressing.APPLIED.items()))
     meta["cost"] = {"rhs_eval": {"flops": sum(c["flops"] for c in group_costs), "groups": group_costs},
                     "Ricci_eval": {"flops": sum(c["flops"] for c in ricci_costs), "groups": ricci_costs},
                     "Hamiltonian_constraint": bd.count_body(h_body),

@@ -113,6 +113,41 @@ __device__ __forceinline__ double nrpy_xshift(const double c, const double h, co
 static __device__ double nrpy_discard[64];
 #define NRPY_OWNED(p) (nrpy_own ? (p) : &nrpy_discard[threadIdx.x & 63])
 
+// Tile-relative addressing (codegen/addressing.py). In a kernel whose block covers a
+// fixed tile and whose accesses lie within NGHOSTS points of (i0, i1, i2), an index
+// splits into a part common to the block (gridfunction offset and the tile's corner,
+// moved back by NGHOSTS in y and z and by XB in x: 64-bit, scalar registers) and a 32-bit
+// byte offset per lane, which is then never negative. A load becomes (scalar base +
+// 32-bit lane offset + constant), one lane offset serving every gridfunction. The
+// bundle's parameter check keeps the lane offset below 2^32 bytes. N0 and N1 are the
+// allocated row and plane sizes (Nxx_plus_2NGHOSTS0/1, or the toolkit's cctk_ash).
+#define NRPY_TILE_ORIGIN_N(XB, N0, N1)                                                         \
+  const int nrpy_o0 = i0 - (int)threadIdx.x - (XB), nrpy_o1 = i1 - (int)threadIdx.y - NGHOSTS, \
+            nrpy_o2 = i2 - (int)threadIdx.z - NGHOSTS;                                         \
+  const uint32_t nrpy_off = 8u * (uint32_t)((i0 - nrpy_o0) + (N0) * ((i1 - nrpy_o1) + (N1) * (i2 - nrpy_o2))); \
+  const ptrdiff_t nrpy_org = (ptrdiff_t)nrpy_o0 + (ptrdiff_t)(N0) * ((ptrdiff_t)nrpy_o1 + (ptrdiff_t)(N1) * nrpy_o2)
+// index B + (point index inside the gridfunction), B the part common to the block (the
+// gridfunction's offset, or 0): B is added first so that the compiler keeps the sum of
+// the uniform parts in scalar registers. x offset as the load's constant offset (origin
+// XB = 0): one lane offset per (y, z) stencil row, and the compiler may fuse loads of
+// x-neighbours
+#define NRPY_TILE_XC(B, N0, N1, i, j, k)                                                       \
+  ((B) + nrpy_org + (ptrdiff_t)((i) - i0) +                                                    \
+   (size_t)((nrpy_off + 8u * (uint32_t)((N0) * (((j) - i1) + (N1) * ((k) - i2)))) >> 3))
+// x offset inside the lane offset (origin XB = NGHOSTS): one lane offset per stencil
+// point, no loads fused
+#define NRPY_TILE_XL(B, N0, N1, i, j, k)                                                       \
+  ((B) + nrpy_org +                                                                            \
+   (size_t)((nrpy_off + 8u * (uint32_t)(((i) - i0) + (N0) * (((j) - i1) + (N1) * ((k) - i2)))) >> 3))
+#define NRPY_TILE_ORIGIN(XB) NRPY_TILE_ORIGIN_N(XB, Nxx_plus_2NGHOSTS0, Nxx_plus_2NGHOSTS1)
+#define NRPY_GF_OFFSET(g) ((size_t)(g) * ((size_t)Nxx_plus_2NGHOSTS0 * Nxx_plus_2NGHOSTS1 * Nxx_plus_2NGHOSTS2))
+#define IDX4_TILE_XC(g, i, j, k) NRPY_TILE_XC(NRPY_GF_OFFSET(g), Nxx_plus_2NGHOSTS0, Nxx_plus_2NGHOSTS1, i, j, k)
+#define IDX4_TILE_XL(g, i, j, k) NRPY_TILE_XL(NRPY_GF_OFFSET(g), Nxx_plus_2NGHOSTS0, Nxx_plus_2NGHOSTS1, i, j, k)
+// the toolkit's access form: one array per gridfunction, rows and planes of cctk_ash
+#define NRPY_ETK_TILE_ORIGIN(XB) NRPY_TILE_ORIGIN_N(XB, cctkGH.ash0, cctkGH.ash1)
+#define CCTK_GFINDEX3D_TILE_XC(GH, i, j, k) NRPY_TILE_XC((ptrdiff_t)0, (GH).ash0, (GH).ash1, i, j, k)
+#define CCTK_GFINDEX3D_TILE_XL(GH, i, j, k) NRPY_TILE_XL((ptrdiff_t)0, (GH).ash0, (GH).ash1, i, j, k)
+
 // Batched launches (nrpy_batch.h): array roles of an instance and index ranges.
 #define NRPY_BATCH_ROLE_YN 0
 #define NRPY_BATCH_ROLE_KODD 1

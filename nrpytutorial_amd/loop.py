@@ -66,7 +66,8 @@ def stage_lds(name, kernel):
     with its mixed-derivative stencils staged in LDS boxes and the x-offset reads of its
     other stencil gridfunctions built by cross-lane moves (codegen/lds.py). A kernel with
     neither keeps its tunable block shape; one with only the moves runs on the default
-    block shape, fixed."""
+    block shape, fixed. A kernel with a fixed block shape may also get tile-relative
+    addressing (codegen/addressing.py)."""
     if not LDS_STAGING:
         return kernel, None
     from .codegen import lds
@@ -82,6 +83,11 @@ def stage_lds(name, kernel):
     if staged is None:
         return kernel, None
     LDS_KERNELS[name] = list(tile)
+    from .codegen import addressing
+    form = addressing.form_for(name)
+    if form:
+        # LDS_STAGING is the bundle's NGHOSTS (FD order / 2 + 1)
+        staged = addressing.tile_address(staged, form, tile, LDS_STAGING, name)
     return staged, "dim3(%d, %d, %d)" % tile
 
 

@@ -7,7 +7,8 @@
   the memory accesses of both are reduced to (gridfunction, RHS or not,
   offset), every statement after the bounds check is identical, LDS staging
   included (as a multiset for a SENR kernel that builds its x-neighbours by lane
-  moves, which also reorders its stencil-level statements). So the GPU parity of the SENR kernels against the oracle carries
+  moves, which also reorders its stencil-level statements; the lines that switch a
+  kernel to tile-relative addressing are left out). So the GPU parity of the SENR kernels against the oracle carries
   over, and the GPU tests check the values as well.
 * The entry points reject bad grids, NULL pointers and FD orders other than 4
   on the host, before anything reaches the GPU.
@@ -80,8 +81,15 @@ def _undo_xshift(body):
                      if not re.match(r"\s*const double (in_gfs\[|nrpy_h\d+ =)", l))
 
 
+_TILE_ADDR = re.compile(r"\s*(#pragma (push|pop)_macro\(\"(IDX4|CCTK_GFINDEX3D)\"\)|#undef (IDX4|CCTK_GFINDEX3D)|"
+                        r"#define (IDX4|CCTK_GFINDEX3D) \w+_TILE_X[CL]|NRPY_(ETK_)?TILE_ORIGIN\(\w+\);)$")
+
+
 def _statements(kernel, bounds):
-    """Statements after the bounds check, accesses reduced to ACC(GF, rhs?, offset)."""
+    """Statements after the bounds check, accesses reduced to ACC(GF, rhs?, offset). The
+    lines that switch a kernel to tile-relative addressing (the same accesses, other
+    address arithmetic) are dropped."""
+    kernel = "\n".join(l for l in kernel.splitlines() if not _TILE_ADDR.match(l))
     body = kernel[kernel.index(bounds) + len(bounds):]
     if "nrpy_xshift<" in body:
         body = _undo_xshift(body)
