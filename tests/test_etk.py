@@ -165,6 +165,8 @@ def test_host_side_errors(baikal):
     assert _call(b, cctk_grid(**good, ash=(19, 18, 16))) == -2                # ash < lsh
     assert _call(b, cctk_grid(**dict(good, delta_space=(0.1, 0.0, 0.1)))) == -4
     assert _call(b, cctk_grid(**good, ash=(1 << 11, 1 << 10, 1 << 10))) == -2  # 2^31 points per array
+    # xy-plane past the bound of the tile-addressed kernels (9.6e8 points, under 2^31)
+    assert _call(b, cctk_grid(**good, ash=(20000, 3000, 16))) == -2
     evol = [0x1000] * len(b.evol)
     evol[5] = 0
     assert _call(b, cctk_grid(**good), evol=evol) == -1
