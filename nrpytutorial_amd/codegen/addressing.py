@@ -33,9 +33,11 @@ a given tile; the bundle's grid check rejects larger grids (TILE_BOUND).
 import os
 import re
 
-# kernel name -> "xc" or "xl"; NRPY_TILE_ADDR ("name=form,...", or "off") overrides for
-# measurements
+# kernel name (or a prefix ending in "*") -> "xc" or "xl"; NRPY_TILE_ADDR
+# ("name=form,...", or "off") overrides for measurements
 TILE_ADDRESSING = {}
+# the bundle's NGHOSTS, the reach of every access the pass accepts
+NGHOSTS = None
 # largest tile (x, y, z) among the rewritten kernels; (0, 0, 0) while none is
 TILE_BOUND = [0, 0, 0]
 # kernel name -> form, of the kernels rewritten so far
@@ -44,11 +46,17 @@ APPLIED = {}
 
 def form_for(name):
     env = os.environ.get("NRPY_TILE_ADDR")
+    table = TILE_ADDRESSING
     if env is not None:
         if env in ("", "off"):
             return None
-        return dict(kv.split("=") for kv in env.split(",")).get(name)
-    return TILE_ADDRESSING.get(name)
+        table = dict(kv.split("=") for kv in env.split(","))
+    if name in table:
+        return table[name]
+    for key, form in table.items():
+        if key.endswith("*") and name.startswith(key[:-1]):
+            return form
+    return None
 
 
 _I2 = re.compile(r"^\s*const int i2 = [^;]*\(int\)threadIdx\.z;$")
@@ -58,9 +66,10 @@ _I01 = (re.compile(r"^\s*const int i0 = [^;]*\(int\)threadIdx\.x;$", re.M),
 # origin macro)
 ACCESS = {"IDX4": (re.compile(r"\bIDX4S?\("), None, "NRPY_TILE_ORIGIN"),
           "CCTK_GFINDEX3D": (re.compile(r"\bCCTK_GFINDEX3D\("), "cctkGH", "NRPY_ETK_TILE_ORIGIN")}
-# macros of the runtime that expand to IDX4 with offsets this pass cannot see (the RK4
-# preload's IDX4(gf, i0, i1, i2) is the centre point and in range)
-_UNSAFE_MACROS = ("VJP_G_AT(",)
+# VJP_G_AT(k, o0, o1, o2) of the runtime expands to IDX4(k, (i0) + (o0), ...) (guarded
+# by an interior test): accepted with literal offsets; the RK4 preload's
+# IDX4(gf, i0, i1, i2) is the centre point and in range
+_VJP = re.compile(r"\bVJP_G_AT\(")
 
 
 def _args(src, start):
@@ -107,10 +116,15 @@ def check_accesses(body, nghosts, access="IDX4"):
     CCTK_GFINDEX3D) in the kernel body is within nghosts points of (i0, i1, i2) along
     each axis."""
     use, first, _ = ACCESS[access]
-    for bad in _UNSAFE_MACROS:
-        if bad in body:
-            raise ValueError("tile addressing: %s in the kernel" % bad.rstrip("("))
+    if access != "IDX4" and _VJP.search(body):
+        raise ValueError("tile addressing: VJP_G_AT in a kernel of the toolkit's access form")
     n = 0
+    for m in _VJP.finditer(body):
+        a = _args(body, m.end())
+        if len(a) != 4 or not re.fullmatch(r"\d+", a[0]) or \
+                not all(re.fullmatch(r"-?\d+", o) and abs(int(o)) <= nghosts for o in a[1:]):
+            raise ValueError("tile addressing: VJP_G_AT(%s) not provably in range" % ", ".join(a))
+        n += 1
     for m in use.finditer(body):
         a = _args(body, m.end())
         if len(a) != 4 or (first and a[0] != first) or not _x_ok(a[1], nghosts) \
@@ -121,11 +135,14 @@ def check_accesses(body, nghosts, access="IDX4"):
     return n
 
 
-def tile_address(kernel_src, form, tile, nghosts, name=None, access="IDX4"):
+def tile_address(kernel_src, form, tile, nghosts=None, name=None, access="IDX4"):
     """The kernel with tile-relative addressing of its accesses through the access macro
     (IDX4, or CCTK_GFINDEX3D), in form "xc" or "xl". tile is the kernel's fixed block
-    shape, nghosts the bundle's NGHOSTS; raises ValueError if the kernel does not have
-    the shape this relies on."""
+    shape, nghosts the bundle's NGHOSTS (default: NGHOSTS); raises ValueError if the
+    kernel does not have the shape this relies on."""
+    nghosts = NGHOSTS if nghosts is None else nghosts
+    if not nghosts:
+        raise ValueError("tile addressing: NGHOSTS unknown")
     if form not in ("xc", "xl"):
         raise ValueError("tile addressing form %r" % (form,))
     lines = kernel_src.split("\n")

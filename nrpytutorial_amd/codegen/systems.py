@@ -693,8 +693,22 @@ BSSN_DIFFERENCED_GFS = "alpha,cf"
 
 
 def VJP_GROUPS(groups):
-    """Output groups of the BSSN RHS VJP's pass 1: the forward output groups."""
-    return [list(grp) for grp in groups]
+    """Output groups of the BSSN RHS VJP's pass 1: the forward output groups, or the
+    prefix groups of NRPY_BSSN_VJP_GROUPS (measurements; same syntax as
+    NRPY_BSSN_RHS_GROUPS). Finer groups spill less but recompute the shared forward
+    temporaries and add to each other's slots: at 96^3 one adjoint RK4 step took 38.6 ms
+    with the forward groups, 42.0 ms with the Lambda-bar group split into betU,vetU and
+    lambdaU, 41.4 ms into betU, vetU and lambdaU, 48.5 ms with lambdaU per component."""
+    import os
+    spec = os.environ.get("NRPY_BSSN_VJP_GROUPS")
+    if not spec:
+        return [list(grp) for grp in groups]
+    names = [n for grp in groups for n in grp]
+    out = [[n for n in names if any(n.startswith(p) for p in g.split(",") if p)] for g in spec.split("|")]
+    out = [g for g in out if g]
+    if sorted(n for g in out for n in g) != sorted(names):
+        raise ValueError("VJP groups do not partition the outputs: " + spec)
+    return out
 
 
 def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
@@ -761,6 +775,7 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     addressing.TILE_ADDRESSING = {}
     addressing.TILE_BOUND[:] = [0, 0, 0]
     addressing.APPLIED.clear()
+    addressing.NGHOSTS = FD_order // 2 + 1
     for k, grp in enumerate(groups):
         if CoordSystem == "Cartesian" and set(grp) == MIN_TWO_WAVES_OUTPUTS:
             bd.lp.MIN_WAVES_PER_EU["rhs_eval_rk4_substep_g%d" % k] = bd.lp.MIN_WAVES_PER_EU["rhs_eval_g%d" % k] = 2
@@ -870,6 +885,12 @@ def bssn(CoordSystem, symmetry_axes, bundle_name, FD_order=4):
     # the Ricci tensor (inputs in_gfs), for adjoints through rhs_eval and Ricci_eval.
     t0 = time.time()
     # Pass 1 runs per output group, as the forward kernels do; pass 2 per cotangent gridfunction.
+    # Tile-relative addressing (Cartesian grids), measured on one 96^3 adjoint RK4 step: 38.5 ms
+    # without; 37.8 with the transposes in form "xc" (38.3 in "xl"); 36.5 with the RHS pass-1
+    # kernels in "xl" as well (41.7 in "xc"); 37.2 with the Ricci pass-1 kernels in "xl" too.
+    if CoordSystem == "Cartesian":
+        addressing.TILE_ADDRESSING.update({"rhs_eval_vjp_transpose_*": "xc", "Ricci_eval_vjp_transpose_*": "xc",
+                                           "rhs_eval_vjp_pointwise_*": "xl"})
     vjp_groups = VJP_GROUPS(groups)
     vjp_src, vslots, _, _ = adj.vjp_kernels("rhs_eval", evol_rhss, betaU, "InteriorPoints,Enable_rfm_precompute",
                                             "rfm_struct *restrict rfmstruct,",

@@ -87,7 +87,7 @@ def stage_lds(name, kernel):
     form = addressing.form_for(name)
     if form:
         # LDS_STAGING is the bundle's NGHOSTS (FD order / 2 + 1)
-        staged = addressing.tile_address(staged, form, tile, LDS_STAGING, name)
+        staged = addressing.tile_address(staged, form, tile, LDS_STAGING, name=name)
     return staged, "dim3(%d, %d, %d)" % tile
 
 
@@ -264,6 +264,14 @@ def kernel_and_launcher(name, desc, args, preloop, body, loopopts, postloop):
     block = None
     if rng == "interior":
         kernel, block = stage_lds(name, kernel)
+    if block is None:
+        from .codegen import addressing
+        form = addressing.form_for(name)
+        if form:
+            # tile-relative addressing needs a fixed tile: the default block shape
+            tile = (BLOCK_X, BLOCK_Y, BLOCK_Z)
+            kernel = addressing.tile_address(kernel, form, tile, name=name)
+            block = "dim3(%d, %d, %d)" % tile
     if rng == "interior":
         n0, n1, n2 = "params->Nxx0", "params->Nxx1", "params->Nxx2"
     else:

@@ -223,9 +223,14 @@ def test_pass_refuses_accesses_out_of_range(acc, ok):
         else:
             with pytest.raises(ValueError):
                 addressing.tile_address(src, "xc", (16, 4, 4), 3)
+        # VJP_G_AT with literal offsets within NGHOSTS is accepted, beyond it refused
+        vjp = "const double b = VJP_G_AT(7, %s);\n    const double a ="
+        if ok:
+            addressing.tile_address(src.replace("const double a =", vjp % "0, -3, 2"), "xc", (16, 4, 4), 3)
         with pytest.raises(ValueError):
-            addressing.tile_address(src.replace("const double a =", "const double b = VJP_G_AT(0, 0, 0, 0);\n"
-                                                "    const double a ="), "xc", (16, 4, 4), 3)
+            addressing.tile_address(src.replace("const double a =", vjp % "4, 0, 0"), "xc", (16, 4, 4), 3)
+        with pytest.raises(ValueError):
+            addressing.tile_address(src.replace("const double a =", vjp % "0, k, 0"), "xc", (16, 4, 4), 3)
     finally:
         addressing.TILE_BOUND[:] = saved[0]
         addressing.APPLIED.clear()
@@ -251,9 +256,15 @@ def test_bundle_kernels_with_tile_addressing(bundle):
         b = kernels[n]
         assert b.count('#pragma push_macro("%s")' % access) == b.count('#pragma pop_macro("%s")' % access) == 1
         assert "#define %s %s_TILE_%s" % (access, access, form.upper()) in b
-    tiles = [meta["lds_kernels"].get(n) for n in named]
     if bundle == "bssn_cartesian":
-        assert all(tiles)                          # fixed block shapes
+        # every launch of a rewritten kernel uses a fixed block shape (not the tunable table)
+        tiles = []
+        for n in named:
+            launcher = re.search(r'extern "C" int %s\((.*?)\n}\n' % n, src, re.S)
+            assert launcher, n
+            block = re.search(r"const dim3 block = dim3\((\d+), (\d+), (\d+)\);", launcher.group(1))
+            assert block and "nrpy_block(" not in launcher.group(1), n
+            tiles.append([int(x) for x in block.groups()])
         bx, by, bz = (max(t[d] for t in tiles) for d in range(3))
         assert ("if ((size_t)(%d + 2*NGHOSTS) + (size_t)p->Nxx_plus_2NGHOSTS0 * ((%d + 2*NGHOSTS) + "
                 "(size_t)p->Nxx_plus_2NGHOSTS1 * (%d + 2*NGHOSTS)) > ((size_t)1 << 29)) return 1;"
