@@ -44,12 +44,20 @@
 
 // Pass-2 helper of the generated vector-Jacobian products: slot k of the pointwise
 // adjoint at the point offset (o0,o1,o2) from (i0,i1,i2), or 0 outside the interior
-// (only interior points carry right-hand sides).
+// (only interior points carry right-hand sides). The load is unconditional and the
+// interior test selects its value: a load under the test compiles to a branch per load,
+// each waiting for its data before the next is issued. Offsets reach at most NGHOSTS (G)
+// beyond the array along each axis; with the plane index held in [1, N2 - 2] every
+// address stays inside slot k's own array (the lowest is N0 (N1 - G) - G >= 0 points in,
+// the highest N0 (N1 - G - 1) - G + 1 > 0 points short of its end, as N1 >= 2 G + 1),
+// and wherever the clamp moves the index the interior test is false.
+__device__ __forceinline__ double nrpy_vjp_select(const bool inside, const double v) { return inside ? v : 0.0; }
 #define VJP_G_AT(k, o0, o1, o2)                                                                \
-  (((i0) + (o0) >= NGHOSTS && (i0) + (o0) < Nxx0 + NGHOSTS && (i1) + (o1) >= NGHOSTS &&       \
-    (i1) + (o1) < Nxx1 + NGHOSTS && (i2) + (o2) >= NGHOSTS && (i2) + (o2) < Nxx2 + NGHOSTS)    \
-       ? vjp_G[IDX4(k, (i0) + (o0), (i1) + (o1), (i2) + (o2))]                               \
-       : 0.0)
+  nrpy_vjp_select((i0) + (o0) >= NGHOSTS && (i0) + (o0) < Nxx0 + NGHOSTS && (i1) + (o1) >= NGHOSTS && \
+                      (i1) + (o1) < Nxx1 + NGHOSTS && (i2) + (o2) >= NGHOSTS &&                  \
+                      (i2) + (o2) < Nxx2 + NGHOSTS,                                              \
+                  vjp_G[IDX4(k, (i0) + (o0), (i1) + (o1),                                        \
+                             min(max((i2) + (o2), 1), Nxx_plus_2NGHOSTS2 - 2))])
 
 // Launch geometry. Kernels run on a 1-D grid of blocks of shape
 // (BX, BY, BZ) tiling the (n0, n1, n2) index box. The hardware hands block b
