@@ -580,7 +580,26 @@ __global__ void nrpy_bc_composed_kernel(int nghost, const int *__restrict__ gdes
   double *g = gfs + (size_t)gf * Ntot;
   const int bit = 1 << gfp.p[gf];
   double acc = 0.0;
-  for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
+  const int b = rowptr[r], e = rowptr[r + 1];
+  // The first four entries load unconditionally (at an index held inside the row) and
+  // each term is added only if its entry exists, so their index loads, then their value
+  // loads, are in flight together: a loop waits twice per entry (most rows hold three).
+  // Longer rows go on in a loop. The sum keeps its order.
+  if (e > b) {
+    double c[4], v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int k = min(b + j, e - 1);
+      c[j] = (negmask[k] & bit) ? -coeff[k] : coeff[k];
+      v[j] = g[src[k]];
+    }
+    // an empty statement that uses every loaded value on every path, after all the loads:
+    // otherwise the compiler moves the loads of entries past a row's end under branches
+    asm volatile("" : : "v"(c[0]), "v"(v[0]), "v"(c[1]), "v"(v[1]), "v"(c[2]), "v"(v[2]), "v"(c[3]), "v"(v[3]));
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc = b + j < e ? fma(c[j], v[j], acc) : acc;   /* acc += c * v, contracted */
+  }
+  for (int k = b + 4; k < e; k++) {
     const double c = (negmask[k] & bit) ? -coeff[k] : coeff[k];
     acc += c * g[src[k]];
   }
