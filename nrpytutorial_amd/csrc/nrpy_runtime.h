@@ -569,6 +569,37 @@ extern "C" int apply_bcs_curvilinear(const paramstruct *params, const bc_struct 
 // ---------------------------------------------------------------------------
 // 3. Composed ghost fill: one launch, reads interior points only.
 
+// Sum over the entries [b, e) of a map row of c_k g[idx_k], in order (acc += c v, the
+// multiply-add contracted), c_k = -coef_k where mask_k has the gridfunction's parity bit,
+// else coef_k. The first four entries load unconditionally (at an index held inside the
+// row) and each term is added only if its entry exists, so their index loads, then their
+// value loads, are in flight together: a loop waits twice per entry (most rows hold three).
+// Longer rows go on in a loop.
+__device__ __forceinline__ double nrpy_map_row_sum(const int b, const int e, const int *__restrict__ idx,
+                                                   const double *__restrict__ coef, const int *__restrict__ mask,
+                                                   const int bit, const double *g) {
+  double acc = 0.0;
+  if (e > b) {
+    double c[4], v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int k = min(b + j, e - 1);
+      c[j] = (mask[k] & bit) ? -coef[k] : coef[k];
+      v[j] = g[idx[k]];
+    }
+    // an empty statement that uses every loaded value on every path, after all the loads:
+    // otherwise the compiler moves the loads of entries past a row's end under branches
+    asm volatile("" : : "v"(c[0]), "v"(v[0]), "v"(c[1]), "v"(v[1]), "v"(c[2]), "v"(v[2]), "v"(c[3]), "v"(v[3]));
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc = b + j < e ? fma(c[j], v[j], acc) : acc;   /* acc += c * v, contracted */
+  }
+  for (int k = b + 4; k < e; k++) {
+    const double c = (mask[k] & bit) ? -coef[k] : coef[k];
+    acc += c * g[idx[k]];
+  }
+  return acc;
+}
+
 __global__ void nrpy_bc_composed_kernel(int nghost, const int *__restrict__ gdest, const int *__restrict__ rowptr,
                                         const int *__restrict__ src, const double *__restrict__ coeff,
                                         const int *__restrict__ negmask, int num_gfs, nrpy_gf_parity_args gfp,
@@ -579,31 +610,7 @@ __global__ void nrpy_bc_composed_kernel(int nghost, const int *__restrict__ gdes
   const int r = (int)(t % nghost);
   double *g = gfs + (size_t)gf * Ntot;
   const int bit = 1 << gfp.p[gf];
-  double acc = 0.0;
-  const int b = rowptr[r], e = rowptr[r + 1];
-  // The first four entries load unconditionally (at an index held inside the row) and
-  // each term is added only if its entry exists, so their index loads, then their value
-  // loads, are in flight together: a loop waits twice per entry (most rows hold three).
-  // Longer rows go on in a loop. The sum keeps its order.
-  if (e > b) {
-    double c[4], v[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int k = min(b + j, e - 1);
-      c[j] = (negmask[k] & bit) ? -coeff[k] : coeff[k];
-      v[j] = g[src[k]];
-    }
-    // an empty statement that uses every loaded value on every path, after all the loads:
-    // otherwise the compiler moves the loads of entries past a row's end under branches
-    asm volatile("" : : "v"(c[0]), "v"(v[0]), "v"(c[1]), "v"(v[1]), "v"(c[2]), "v"(v[2]), "v"(c[3]), "v"(v[3]));
-#pragma unroll
-    for (int j = 0; j < 4; j++) acc = b + j < e ? fma(c[j], v[j], acc) : acc;   /* acc += c * v, contracted */
-  }
-  for (int k = b + 4; k < e; k++) {
-    const double c = (negmask[k] & bit) ? -coeff[k] : coeff[k];
-    acc += c * g[src[k]];
-  }
-  g[gdest[r]] = acc;
+  g[gdest[r]] = nrpy_map_row_sum(rowptr[r], rowptr[r + 1], src, coeff, negmask, bit, g);
 }
 
 extern "C" int apply_bcs_curvilinear_composed(const paramstruct *params, const bc_struct *bc, const int NUM_GFS,
@@ -639,12 +646,7 @@ __global__ void nrpy_bc_transpose_kernel(int ntsrc, const int *__restrict__ tsrc
   const int r = (int)(t % ntsrc);
   double *g = gfs + (size_t)gf * Ntot;
   const int bit = 1 << gfp.p[gf];
-  double acc = 0.0;
-  for (int k = trowptr[r]; k < trowptr[r + 1]; k++) {
-    const double c = (tnegmask[k] & bit) ? -tcoeff[k] : tcoeff[k];
-    acc += c * g[tghost[k]];
-  }
-  g[tsrc[r]] += acc;
+  g[tsrc[r]] += nrpy_map_row_sum(trowptr[r], trowptr[r + 1], tghost, tcoeff, tnegmask, bit, g);
 }
 
 __global__ void nrpy_clear_ghosts_kernel(int nghost, const int *__restrict__ gdest, int num_gfs, size_t Ntot,
