@@ -72,6 +72,11 @@ struct bc_struct {
   int8_t *d_ib_parity = nullptr;   // 10 per inner point
   int ob_off[NGHOSTS + 1];
   int ib_off[NGHOSTS + 1];
+  // the i0-face points of each layer whose (i1, i2) is not interior (the Sommerfeld
+  // condition takes the interior rows' i0-face points in one tiled launch)
+  int *d_ob_xedge_dest = nullptr;
+  int8_t *d_ob_xedge_face = nullptr;
+  int ob_xedge_off[NGHOSTS + 1];
   // composed ghost map (CSR over ghost points)
   int nghost = 0, nterms = 0;
   int *d_gdest = nullptr;
@@ -343,6 +348,8 @@ extern "C" void bcstruct_free(bc_struct *bc) {
   (void)hipFree(bc->d_ob_dest);
   (void)hipFree(bc->d_ob_stride);
   (void)hipFree(bc->d_ob_face);
+  (void)hipFree(bc->d_ob_xedge_dest);
+  (void)hipFree(bc->d_ob_xedge_face);
   (void)hipFree(bc->d_ib_dest);
   (void)hipFree(bc->d_ib_src);
   (void)hipFree(bc->d_ib_parity);
@@ -415,11 +422,21 @@ extern "C" int bcstruct_create(const paramstruct *params, double *xx[3], bc_stru
   if (rc) return rc;
   bc_struct *bc = *out;
   *out = nullptr;
-  std::vector<int> obd, obs, ibd, ibs;
-  std::vector<int8_t> ibp, obf;
-  const int N0 = bc->N[0], N1 = bc->N[1];
-  bc->ob_off[0] = bc->ib_off[0] = 0;
+  std::vector<int> obd, obs, ibd, ibs, oxd;
+  std::vector<int8_t> ibp, obf, oxf;
+  const int N0 = bc->N[0], N1 = bc->N[1], N2 = bc->N[2];
+  bc->ob_off[0] = bc->ib_off[0] = bc->ob_xedge_off[0] = 0;
   for (int gz = 0; gz < NGHOSTS; gz++) {
+    for (int q = 0; q < bc->ob_dir_end[gz][0]; q++) {
+      const outer_bc &o = bc->outer[gz][q];
+      const int i1 = o.outer_bc_dest_pt.i1, i2 = o.outer_bc_dest_pt.i2;
+      if (i1 >= NGHOSTS && i1 < N1 - NGHOSTS && i2 >= NGHOSTS && i2 < N2 - NGHOSTS) continue;
+      oxd.push_back(o.outer_bc_dest_pt.i0 + N0 * (i1 + N1 * i2));
+      oxf.push_back(o.FACEi0);
+      oxf.push_back(o.FACEi1);
+      oxf.push_back(o.FACEi2);
+    }
+    bc->ob_xedge_off[gz + 1] = (int)oxd.size();
     for (const outer_bc &o : bc->outer[gz]) {
       obd.push_back(o.outer_bc_dest_pt.i0 + N0 * (o.outer_bc_dest_pt.i1 + N1 * o.outer_bc_dest_pt.i2));
       obs.push_back(o.FACEi0 + N0 * (o.FACEi1 + N1 * o.FACEi2));
@@ -438,6 +455,8 @@ extern "C" int bcstruct_create(const paramstruct *params, double *xx[3], bc_stru
   bool ok = nrpy_upload(&bc->d_ob_dest, obd.data(), obd.size()) &&
             nrpy_upload(&bc->d_ob_stride, obs.data(), obs.size()) &&
             nrpy_upload(&bc->d_ob_face, obf.data(), obf.size()) &&
+            nrpy_upload(&bc->d_ob_xedge_dest, oxd.data(), oxd.size()) &&
+            nrpy_upload(&bc->d_ob_xedge_face, oxf.data(), oxf.size()) &&
             nrpy_upload(&bc->d_ib_dest, ibd.data(), ibd.size()) &&
             nrpy_upload(&bc->d_ib_src, ibs.data(), ibs.size()) &&
             nrpy_upload(&bc->d_ib_parity, ibp.data(), ibp.size()) &&
