@@ -703,6 +703,11 @@ def main():
                   "kernel_ms": {"Ricci_eval": 1e3 * t_ricci, "rhs_eval": 1e3 * t_rhs, "ghost_fill": 1e3 * t_bc,
                                 "detg": 1e3 * t_detg,
                                 **{"rhs_eval_rk4_substep_g%d" % k: 1e3 * t for k, t in enumerate(t_group)}},
+                  "kernel_ms_note": "each figure is one kernel launched eagerly %d times back to back on the launch "
+                                    "stream, HIP events around all of them, divided by %d: it includes the gaps "
+                                    "between eager launches, which the graph-replayed step (ms_per_step) does not "
+                                    "have, so 4 x the per-substep sum exceeds ms_per_step by about that overhead"
+                                    % (reps, reps),
                   "fused_substep_fp64_TFLOPs": fused_flops * npts / t_fused / 1e12,
                   "fused_flops_per_point": fused_flops, "ricci_flops_per_point": cost["Ricci_eval"]["flops"],
                   "rhs_groups": groups, "bl_72x12x2_50steps": bl, "adjoint_scalarwave_128": adjoint,
