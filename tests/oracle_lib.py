@@ -17,9 +17,11 @@ _DPP = ctypes.POINTER(ctypes.c_void_p)
 
 
 class Oracle:
-    def __init__(self, bundle, fast=False):
+    def __init__(self, bundle, fast=False, variant=None):
+        """variant: None, or "asan" for the sanitizer build (oracle/Makefile target asan)."""
         self.meta = json.load(open(os.path.join(GEN, bundle + ".json")))
-        path = os.path.join(GEN, ("liboracle_fast_%s.so" if fast else "liboracle_%s.so") % bundle)
+        kind = variant + "_" if variant else "fast_" if fast else ""
+        path = os.path.join(GEN, "liboracle_%s%s.so" % (kind, bundle))
         if not os.path.exists(path):
             raise RuntimeError("oracle library %s missing: build first" % path)
         self.lib = ctypes.CDLL(path)
