@@ -68,7 +68,10 @@ int MoL_step_forward_in_time_sommerfeld_unfused(const paramstruct *params, const
 
 /* Substep S (0-3) of the fused Sommerfeld step at the interior: the RHS and its RK4 update
  * (as rhs_eval_rk4_substep), and at the interior points next to the boundary the RHS also
- * into shell_rhs (NUM_EVOL_GFS gridfunctions, whole grid). No counterpart in the reference. */
+ * into shell_rhs: NUM_EVOL_GFS whole-grid gridfunctions (points next to an i1 or i2 boundary),
+ * followed by NUM_EVOL_GFS x N2 x N1 x 2 doubles [gf][i2][i1][side] (points next to an i0
+ * boundary; side 0 at i0 = NGHOSTS, 1 at i0 = N0 - 1 - NGHOSTS), N = Nxx_plus_2NGHOSTS.
+ * apply_bcs_sommerfeld_rk4 reads R in this layout. No counterpart in the reference. */
 int rhs_eval_rk4_substep_shell(const int S, const paramstruct *params, const double *in_gfs, double *y_n,
                                double *rt, double *out, double *shell_rhs, const double dt,
                                hipStream_t stream);

@@ -346,9 +346,19 @@ def _fused_shell_kernel(fused_kernel, name):
     k = k.replace("const double dt) {", "const double dt, double *__restrict__ shell_rhs) {", 1)
     stores = _re.findall(r"NRPY_RK4_STORE\(S, \d+, __rhs_(\w+)\);", k)
     last = [m.end() for m in _re.finditer(r"NRPY_RK4_STORE\(S, \d+, __rhs_\w+\);", k)][-1]
-    block = ("\n  if (i0 == NGHOSTS || i0 == Nxx0 + NGHOSTS - 1 || i1 == NGHOSTS || i1 == Nxx1 + NGHOSTS - 1 ||"
-             " i2 == NGHOSTS || i2 == Nxx2 + NGHOSTS - 1) {\n" +
-             "".join("    shell_rhs[IDX4(%sGF, i0, i1, i2)] = __rhs_%s;\n" % (g, g) for g in stores) + "  }")
+    # points next to an i1 or i2 boundary: in the grid layout, where the i1- and i2-face kernels
+    # read them. Points next to an i0 boundary: also (or only) in a compact array after the
+    # NUM_EVOL_GFS grids, [gf][i2][i1][side], read by the i0-face launch; there a wave's lanes
+    # and neighbouring blocks fill whole cache lines, where in the grid layout each value of
+    # such a point sits on a line of its own.
+    block = ("\n  if (i1 == NGHOSTS || i1 == Nxx1 + NGHOSTS - 1 || i2 == NGHOSTS || i2 == Nxx2 + NGHOSTS - 1) {\n" +
+             "".join("    shell_rhs[IDX4(%sGF, i0, i1, i2)] = __rhs_%s;\n" % (g, g) for g in stores) + "  }\n"
+             "  if (i0 == NGHOSTS || i0 == Nxx0 + NGHOSTS - 1) {\n"
+             "    double *__restrict__ xb = shell_rhs + (size_t)NUM_EVOL_GFS * Nxx_plus_2NGHOSTS0 * "
+             "Nxx_plus_2NGHOSTS1 * Nxx_plus_2NGHOSTS2 + 2 * ((size_t)i1 + (size_t)Nxx_plus_2NGHOSTS1 * i2) + "
+             "(i0 == NGHOSTS ? 0 : 1);\n" +
+             "".join("    xb[(size_t)%sGF * 2 * Nxx_plus_2NGHOSTS1 * Nxx_plus_2NGHOSTS2] = __rhs_%s;\n" % (g, g)
+                     for g in stores) + "  }")
     return k[:last] + block + k[last:]
 
 

@@ -374,7 +374,9 @@ extern "C" void bcstruct_free(bc_struct *bc) {
 // the result is nullptr and the caller takes the unfused step, which needs no scratch.
 static double *nrpy_sommerfeld_scratch(const bc_struct *bc, const int num_gfs, hipStream_t stream) {
   bc_struct *b = const_cast<bc_struct *>(bc);
-  const size_t n = (size_t)num_gfs * bc->N[0] * bc->N[1] * bc->N[2];
+  // num_gfs grids, then the compact RHS at the points next to the i0 boundaries,
+  // [gf][i2][i1][side] (side 0 at i0 = NGHOSTS, 1 at i0 = N0 - 1 - NGHOSTS)
+  const size_t n = (size_t)num_gfs * bc->N[0] * bc->N[1] * bc->N[2] + (size_t)num_gfs * 2 * bc->N[1] * bc->N[2];
   if (b->d_som_rhs && b->som_rhs_n >= n) return b->d_som_rhs;
   if (b->d_som_rhs) return nullptr;              // never resized: a captured step may hold it
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
